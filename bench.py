@@ -1,0 +1,200 @@
+"""Benchmark: secondary-ray throughput of the voxel-traversal lighting path on one MI355X.
+
+Workload (default): 512^3 terrain + 2000 boxes, 2 coarse occupancy levels, one
+1920x1080 view; per hit pixel 1 shadow ray (maxT 128), 16 AO rays (length 32)
+and 1 GGX specular-occlusion ray (maxT 256), all through the DDA marcher
+(raycastShadowVolume semantics, bit-exact). A step is one vx_trace launch over
+the view; inputs (volume, levels, G-buffer) are resident in HBM before timing.
+
+Prints one JSON line. Extra fields: the reference-marcher variant of the same
+frame, the backward (shading adjoint) rate, rebuild and primary-hit times,
+the HBM roofline of the trace kernel and the CPU oracle baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PIXEL_IO_BYTES = 34   # hit pos 12 + normal 12 + material 1 in; shadow 1 + AO 4 + spec 4 out
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
+    """Oracle (C restatement, OpenMP over pixels) on a bounded, deterministic
+    pixel subsample: every k-th hit pixel, sized from a pilot run to ~budget_s."""
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    hit = np.nonzero(mat > 0)[0].astype(np.int32)
+    pilot = hit[:: max(1, hit.size // 2048)]
+    t0 = time.perf_counter()
+    O.trace(occ_host, vd, params, pos, nrm, mat, pal, pixels=pilot, nthreads=cores)
+    dt = time.perf_counter() - t0
+    n_target = int(min(hit.size, max(pilot.size, pilot.size * budget_s / max(dt, 1e-6))))
+    stride = max(1, hit.size // n_target)
+    sample = np.ascontiguousarray(hit[::stride])
+    t0 = time.perf_counter()
+    O.trace(occ_host, vd, params, pos, nrm, mat, pal, pixels=sample, nthreads=cores)
+    dt = time.perf_counter() - t0
+    rays = sample.size * (params["n_shadow"] + params["n_ao"] + params["n_spec"])
+    return {"value": rays / dt / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
+            "sample": f"every {stride}th hit pixel ({sample.size} px, {rays} rays, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline trace (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from voxelengine_amd import api, configs, views
+
+    cfg = configs.CONFIGS["F"]
+    W, H = cfg["width"], cfg["height"]
+    params = configs.trace_params(**cfg["params"])
+    ctx = api.Context(0)
+    stream = torch.cuda.current_stream()
+
+    # ---- scene: generate, mips, rebuild occupancy + levels (untimed setup, timed separately)
+    nx, ny, nz = cfg["dims"]
+    model = api.Model(ctx, nx, ny, nz)
+    model.generate(seed=configs.SEED, terrain_fill=cfg["fill"], nboxes=cfg["nboxes"], box_min=cfg["box"][0],
+                   box_max=cfg["box"][1], grounded=cfg["grounded"])
+    model.build_mips()
+    occ = api.Volume(ctx, nx // 2, ny // 2, nz // 2, nlevels=cfg["levels"])
+    inst = [api.make_instance(model.mip_ptr(0), model.dims, api.IDENTITY, api.IDENTITY)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    occ.clear()
+    occ.rebuild(inst)
+    occ.build_levels()
+    torch.cuda.synchronize()
+    rebuild_ms = (time.perf_counter() - t0) * 1e3
+
+    vd = views.scene_view(cfg["dims"], W, H)
+    g = api.GBuffer(W, H)
+    api.primary(ctx, model, vd, g)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(3):
+        api.primary(ctx, model, vd, g)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    primary_ms = e0.elapsed_time(e1) / 3
+
+    pal = configs.make_palette()
+    pal_d = torch.from_numpy(pal).cuda()
+    mat = g.material.cpu().numpy()
+    hit_px = int((mat > 0).sum())
+    rays_per_frame = hit_px * configs.rays_per_pixel(params)
+
+    # algorithmic bytes: reference fetch count of every ray (one counting launch)
+    vis = api.Visibility(W, H, fetches=True)
+    api.trace(ctx, occ, vd, g, pal_d, params, vis)
+    torch.cuda.synchronize()
+    s_ref = int(vis.fetches.to(torch.int64).sum().item())
+    alg_bytes = s_ref + PIXEL_IO_BYTES * hit_px
+    vis_t = api.Visibility(W, H, fetches=False)
+
+    # ---- timed region: K trace launches on the current stream
+    for _ in range(args.warmup):
+        api.trace(ctx, occ, vd, g, pal_d, params, vis_t)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        api.trace(ctx, occ, vd, g, pal_d, params, vis_t)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = rays_per_frame / (ms_per_step * 1e-3) / 1e9
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "Grays/s (shadow+AO+spec-occl) at 1080p/512^3",
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "F: 512^3 terrain+2000 boxes, 2 levels, 1920x1080, 1 shadow + 16 AO(32) + "
+                               "1 GGX spec-occ ray per hit pixel, DDA marcher",
+                   "hit_pixels": hit_px, "rays_per_frame": rays_per_frame, "marcher": "dda"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_trace", "kernel_ms": round(kernel_ms, 4), "alg_bytes_per_launch": alg_bytes,
+                     "ref_fetches_per_launch": s_ref},
+        "rebuild_ms": round(rebuild_ms, 3),
+        "primary_ms": round(primary_ms, 4),
+    }
+
+    if not args.no_extras:
+        # same frame through the reference's enabled marchers (Sparse / SuperSparse / Sparse)
+        p_ref = configs.trace_params(**dict(cfg["params"], march="reference"))
+        for _ in range(2):
+            api.trace(ctx, occ, vd, g, pal_d, p_ref, vis_t)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(10):
+            api.trace(ctx, occ, vd, g, pal_d, p_ref, vis_t)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        out["reference_marchers_grays_per_s"] = round(rays_per_frame / (ev0.elapsed_time(ev1) / 10 * 1e-3) / 1e9, 4)
+
+        # backward: shading adjoint over the same frame (visibility fixed)
+        light = configs.default_light()
+        rgb = torch.zeros((W * H, 3), dtype=torch.float32, device="cuda")
+        api.shade_forward(ctx, vd, g, vis, pal_d, light, rgb)
+        dl = torch.randn((W * H, 3), dtype=torch.float32, device="cuda")
+        dpal = torch.zeros((256, 6), dtype=torch.float32, device="cuda")
+        dlight = torch.zeros(6, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            api.shade_backward(ctx, vd, g, vis, pal_d, light, dl, dpal, dlight)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(20):
+            api.shade_backward(ctx, vd, g, vis, pal_d, light, dl, dpal, dlight)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        bwd_ms = ev0.elapsed_time(ev1) / 20
+        out["backward"] = {"ms": round(bwd_ms, 4), "grays_per_s": round(rays_per_frame / (bwd_ms * 1e-3) / 1e9, 2),
+                           "px_per_s": round(hit_px / (bwd_ms * 1e-3), 1)}
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        occ_h = occ.download()
+        out["cpu_baseline"] = cpu_baseline(occ_h, vd, params, g.pos.cpu().numpy(), g.normal.cpu().numpy(), mat, pal)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * vx_api.h -- C ABI of the MI355X (gfx950) voxel-traversal lighting path.
+ *
+ * Drop-in replacement for VoxelEngine's voxelized-scene rebuild and the voxel
+ * ray marching behind its shadow / ambient-occlusion / specular-occlusion
+ * passes. Each entry point names the reference interface it replaces
+ * (paths relative to the VoxelEngine source tree).
+ *
+ * Conventions
+ *  - Plain C, POD structs, no exceptions across the ABI. Every call returns
+ *    vx_status: 0 on success, a negative VX_ERR_* code on failure; the message
+ *    of the last failure on the calling thread is vx_last_error().
+ *  - Device pointers (suffix _dev or documented as device) are HIP device
+ *    memory owned by the caller; host pointers are owned by the caller.
+ *    Volumes and models are opaque handles owned by the library.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream). Work
+ *    is stream-ordered; calls return once the work is enqueued unless noted.
+ *  - A context is used by one host thread at a time.
+ *  - Matrices are column-major 4x4 (glm layout). World units are the
+ *    reference's (one world unit = ten fine voxels).
+ */
+#ifndef VX_API_H
+#define VX_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VX_API_VERSION 1
+
+typedef int32_t vx_status;
+enum {
+    VX_OK = 0,
+    VX_ERR_INVALID_ARGUMENT = -1,
+    VX_ERR_OUT_OF_MEMORY = -2,
+    VX_ERR_DEVICE = -3,       /* no usable device or a HIP runtime failure */
+    VX_ERR_UNSUPPORTED = -4,
+    VX_ERR_RANGE = -5,        /* a size beyond a documented limit */
+    VX_ERR_STATE = -6
+};
+
+/* Marchers. DDA = raycastShadowVolume (Light.frag:29-81); SPARSE =
+ * raycastShadowVolumeSparse (Light.frag:131-173); SUPERSPARSE =
+ * raycastShadowVolumeSuperSparse (Light.frag:175-217). */
+enum { VX_MARCH_DDA = 0, VX_MARCH_SPARSE = 1, VX_MARCH_SUPERSPARSE = 2 };
+
+/* Instance change kinds. UPDATE = ShadowVoxSystem::OnUpdate for an entity with
+ * the Changed tag (ShadowVoxSystem.cpp:119-191); DESTROY =
+ * ShadowVoxSystem::OnVoxDestroyed (ShadowVoxSystem.cpp:7-53). */
+enum { VX_INST_UPDATE = 1, VX_INST_DESTROY = 2 };
+
+typedef struct vx_ctx vx_ctx;
+typedef struct vx_volume vx_volume;
+typedef struct vx_model vx_model;
+typedef struct vx_batch vx_batch;
+
+/* evk ImageRegion (Vendor/evk/evk.h:152-155) in texels of the occupancy volume. */
+typedef struct {
+    int32_t x, y, z;
+    uint32_t w, h, d;
+} vx_region;
+
+/* One changed entity: VoxRenderer {Vox, Pivot} + Transform {PreviousWorldMatrix,
+ * WorldMatrix} (Components.h:30-35,60-67). model is a DEVICE pointer to the
+ * model's X-fastest material bytes (VoxAsset::PixelAt, VoxAsset.h:52-56). */
+typedef struct {
+    const uint8_t* model;
+    int32_t sx, sy, sz;
+    float prev[16];
+    float cur[16];
+    float pivot[3];
+    int32_t flags;     /* VX_INST_UPDATE or VX_INST_DESTROY */
+} vx_instance;
+
+/* ViewData (Sources/Graphics/Renderer/View.h:16-31; Common.frag:45-61) plus the
+ * pixel grid it is rendered at. */
+typedef struct {
+    float last_view[16];
+    float view[16];
+    float inv_view[16];
+    float proj[16];
+    float inv_proj[16];
+    float res[2];
+    float ires[2];
+    float cam_pos[3];
+    float jitter[2];
+    int32_t frame;
+    int32_t width, height;
+} vx_view;
+
+/* Ray set-up of the lighting passes. Reference defaults: shadow SPARSE to 128
+ * (LightAmbient.frag:167), AO SUPERSPARSE (LightAmbient.frag:121), spec-occ
+ * SPARSE to 256 (LightReflection.frag:113). */
+typedef struct {
+    int32_t shadow_march, ao_march, spec_march;
+    int32_t n_shadow, n_ao, n_spec;     /* rays per pixel: 0/1, 0..64, 0/1 */
+    float shadow_maxt, ao_maxt, spec_maxt;
+    float light_dir[3];                 /* direction toward the light (SUN_DIR) */
+    uint32_t seed;
+    int32_t frame;
+    int32_t jitter_shadow;              /* 1: soft-shadow jitter of LightAmbient.frag:151-156 */
+    int32_t pad;
+} vx_trace_params;
+
+/* G-buffer (device). Replaces the Geometry pass attachments (Graphics.h:46-60)
+ * with fp32 hit points in fine-voxel units. material 0 = no hit. */
+typedef struct {
+    float* pos;          /* width*height*3 */
+    float* normal;       /* width*height*3 */
+    uint8_t* material;   /* width*height */
+} vx_gbuf;
+
+/* Per-pixel visibility (device): shadow 1 = lit; ao = mean (t/cap)^2 over the
+ * AO rays; spec = reflection-ray distance / spec_maxt (1 = unoccluded).
+ * fetches (optional, may be NULL): occupancy-byte fetches the reference
+ * algorithm performs for the pixel's rays. */
+typedef struct {
+    uint8_t* shadow;
+    float* ao;
+    float* spec;
+    uint32_t* fetches;
+} vx_trace_out;
+
+/* Explicit-ray result, 48 bytes. */
+typedef struct {
+    int32_t hit;       /* 1 hit */
+    int32_t term;      /* DDA: 0 loop exhausted, 1 hit, 2 out of bounds; sparse: 0 miss, 1 hit */
+    int32_t steps;     /* DDA: reference step count nt (x4 restart included); sparse: iterations */
+    int32_t fetches;   /* occupancy-byte fetches of the reference algorithm */
+    int32_t voxel[3];  /* DDA hit voxel (fine units) */
+    float t;           /* DDA: best_t at the hit; sparse: returned distance */
+    float hitpos[3];   /* DDA: origin + dir*best_t */
+    float normal[3];   /* DDA: -sign(dir)*select */
+} vx_ray_result;
+
+/* Synthetic scene description (build-defined inputs for tests and benchmarks). */
+typedef struct {
+    uint32_t seed;
+    float terrain_fill;     /* target fill of the fBm heightfield, 0 = no terrain */
+    int32_t nboxes;
+    int32_t box_min, box_max;
+    int32_t boxes_grounded;
+} vx_scene_desc;
+
+/* ---------------------------------------------------------------- context */
+int32_t vx_api_version(void);
+const char* vx_last_error(void);
+/* Replaces Graphics module start-up (Graphics.cpp) for this path: selects the
+ * HIP device and allocates per-device state. */
+vx_status vx_ctx_create(int32_t device, vx_ctx** out);
+vx_status vx_ctx_destroy(vx_ctx* ctx);
+
+/* ---------------------------------------------------------------- models
+ * VoxAsset (VoxAsset.h:8-62): R8 material volume with 3 mips. Dims are rounded
+ * up to multiples of 4 (VoxAsset.h:27-29). */
+vx_status vx_model_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32_t sz, vx_model** out);
+vx_status vx_model_destroy(vx_model* m);
+vx_status vx_model_dims(const vx_model* m, int32_t dims[3]);
+/* Device pointer of mip level (0..2), X-fastest bytes of dims>>mip. */
+vx_status vx_model_mip_ptr(const vx_model* m, int32_t mip, uint8_t** dev_ptr);
+/* Host upload of mip 0 (VoxAsset::Upload's first-level copy, VoxAsset.cpp:9-13). */
+vx_status vx_model_upload(vx_model* m, const uint8_t* host, size_t nbytes, void* stream);
+/* Download mip level (0..2) to host, nbytes = product of dims>>mip. Synchronous. */
+vx_status vx_model_download(const vx_model* m, int32_t mip, uint8_t* host, size_t nbytes, void* stream);
+/* Mip build: first non-zero child, VoxAsset::Upload (VoxAsset.cpp:16-53). */
+vx_status vx_model_build_mips(vx_model* m, void* stream);
+/* Fill mip 0 with a synthetic terrain/box scene (build-defined). */
+vx_status vx_model_generate(vx_model* m, const vx_scene_desc* desc, void* stream);
+
+/* ---------------------------------------------------------------- occupancy volume
+ * ShadowVoxSystem's volume (ShadowVoxSystem.cpp:55-79): texel dims sx,sy,sz;
+ * each texel a 2x2x2 fine-voxel bitmask. nlevels coarse occupancy levels
+ * (0..3) of 8^3, 64^3 and 512^3 fine-voxel cells. Limits: texel dims <= 4096. */
+vx_status vx_volume_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32_t sz, int32_t nlevels,
+                           vx_volume** out);
+vx_status vx_volume_destroy(vx_volume* v);
+vx_status vx_volume_dims(const vx_volume* v, int32_t dims[3], int32_t* nlevels);
+vx_status vx_volume_clear(vx_volume* v, void* stream);
+/* Upload texel regions from a host copy of the whole volume in the reference
+ * byte layout (X-fastest); replaces the staging copy of ShadowVoxSystem.cpp:193-200
+ * (evk.cpp:759-780). Synchronous. */
+vx_status vx_volume_upload_region(vx_volume* v, const uint8_t* host_bytes, const vx_region* regions,
+                                  uint32_t nregions, void* stream);
+/* Download the whole volume in the reference byte layout. Synchronous. */
+vx_status vx_volume_download(const vx_volume* v, uint8_t* host_bytes, size_t nbytes, void* stream);
+/* ShadowVoxSystem::OnUpdate / OnVoxDestroyed for n changed instances, in order.
+ * Writes one dirty region per instance whose footprint reached the volume
+ * (ShadowVoxSystem.cpp:180-190) to dirty (capacity n). Blocks until the
+ * regions are known. */
+vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, uint32_t n, int32_t occ_threshold,
+                            vx_region* dirty, uint32_t* ndirty, void* stream);
+/* Rebuild the coarse occupancy levels over the given texel regions (NULL = all). */
+vx_status vx_volume_build_levels(vx_volume* v, const vx_region* regions, uint32_t nregions, void* stream);
+/* Download level (1..nlevels) occupancy words (u64, 4x4x4 cells each). Synchronous. */
+vx_status vx_volume_download_level(const vx_volume* v, int32_t level, uint64_t* host, size_t nwords,
+                                   size_t* words_total, void* stream);
+
+/* ---------------------------------------------------------------- passes
+ * Primary hit: GeometryVoxelPipeline::Use (GeometryVoxelPipeline.h:48-70) +
+ * GeometryVoxel.frag main/intersectVolume (GeometryVoxel.frag:63-183) for the
+ * world model with an identity WorldMatrix. */
+vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_view* view, const vx_gbuf* gbuf,
+                     void* stream);
+/* Shadow + AO + spec-occ rays: LightAmbientPipeline::Use (LightAmbientPipeline.h:35)
+ * and LightReflectionPipeline::Use (LightReflectionPipeline.h:34) ray set-up and
+ * marching. palette_dev: 256*6 floats {albedo rgb, roughness, metallic, emissive}. */
+vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
+                   const float* palette_dev, const vx_trace_params* params, const vx_trace_out* out,
+                   void* stream);
+/* Explicit rays (device, n*7 floats: origin3, dir3, maxt) through one marcher. */
+vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const float* rays_dev, uint32_t n,
+                        int32_t march, vx_ray_result* out_dev, void* stream);
+/* Shaded radiance: LightAmbient.frag:178-214 with PBR.frag:3-68, visibility
+ * given. light: host {dir3, rgb3}. ssao_dev may be NULL (1.0). rgb_dev: npx*3. */
+vx_status vx_shade_forward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbuf,
+                           const vx_trace_out* vis, const float* ssao_dev, const float* palette_dev,
+                           const float light[6], float* rgb_dev, void* stream);
+/* Adjoint of vx_shade_forward with visibility fixed. dl_drgb_dev: npx*3.
+ * Writes dpalette_dev (256*6) and dlight_dev (6); deterministic. */
+vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbuf,
+                            const vx_trace_out* vis, const float* ssao_dev, const float* palette_dev,
+                            const float light[6], const float* dl_drgb_dev, float* dpalette_dev,
+                            float* dlight_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

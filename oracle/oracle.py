@@ -1,0 +1,337 @@
+"""ctypes bindings for the oracle (oracle/libvx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the CPU restatement that checks the HIP product.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module. The product package never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvx_oracle.so")
+
+MARCH_DDA, MARCH_SPARSE, MARCH_SUPERSPARSE = 0, 1, 2
+INST_UPDATE, INST_DESTROY = 1, 2
+
+
+class Occ(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("sx", C.c_int32), ("sy", C.c_int32), ("sz", C.c_int32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("model", C.c_void_p), ("sx", C.c_int32), ("sy", C.c_int32), ("sz", C.c_int32),
+                ("prev", C.c_float * 16), ("cur", C.c_float * 16), ("pivot", C.c_float * 3),
+                ("flags", C.c_int32)]
+
+
+class RayResult(C.Structure):
+    _fields_ = [("hit", C.c_int32), ("term", C.c_int32), ("steps", C.c_int32), ("fetches", C.c_int32),
+                ("voxel", C.c_int32 * 3), ("t", C.c_float), ("hitpos", C.c_float * 3),
+                ("normal", C.c_float * 3)]
+
+
+RAY_RESULT_DTYPE = np.dtype([("hit", "<i4"), ("term", "<i4"), ("steps", "<i4"), ("fetches", "<i4"),
+                             ("voxel", "<i4", 3), ("t", "<f4"), ("hitpos", "<f4", 3),
+                             ("normal", "<f4", 3)])
+assert RAY_RESULT_DTYPE.itemsize == C.sizeof(RayResult)
+
+
+class View(C.Structure):
+    _fields_ = [("last_view", C.c_float * 16), ("view", C.c_float * 16), ("inv_view", C.c_float * 16),
+                ("proj", C.c_float * 16), ("inv_proj", C.c_float * 16), ("res", C.c_float * 2),
+                ("ires", C.c_float * 2), ("cam_pos", C.c_float * 3), ("jitter", C.c_float * 2),
+                ("frame", C.c_int32), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("shadow_march", C.c_int32), ("ao_march", C.c_int32), ("spec_march", C.c_int32),
+                ("n_shadow", C.c_int32), ("n_ao", C.c_int32), ("n_spec", C.c_int32),
+                ("shadow_maxt", C.c_float), ("ao_maxt", C.c_float), ("spec_maxt", C.c_float),
+                ("light_dir", C.c_float * 3), ("seed", C.c_uint32), ("frame", C.c_int32),
+                ("jitter_shadow", C.c_int32), ("pad", C.c_int32)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        f32p = C.POINTER(C.c_float)
+        L.vxo_f2i.argtypes = [C.c_float]; L.vxo_f2i.restype = C.c_int32
+        L.vxo_sign.argtypes = [C.c_float]; L.vxo_sign.restype = C.c_float
+        L.vxo_mod.argtypes = [C.c_float, C.c_float]; L.vxo_mod.restype = C.c_float
+        L.vxo_sincos.argtypes = [C.c_float, f32p, f32p]; L.vxo_sincos.restype = None
+        L.vxo_hash.argtypes = [C.c_uint32]; L.vxo_hash.restype = C.c_uint32
+        L.vxo_rand.argtypes = [C.c_uint32, C.c_uint32]; L.vxo_rand.restype = C.c_float
+        L.vxo_pixkey.argtypes = [C.c_uint32, C.c_int32, C.c_uint32]; L.vxo_pixkey.restype = C.c_uint32
+        L.vxo_cosine_hemisphere.argtypes = [C.c_float, C.c_float, f32p]; L.vxo_cosine_hemisphere.restype = None
+        L.vxo_set_volume_at.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.vxo_translate.argtypes = [f32p, f32p, f32p]
+        L.vxo_rebuild.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Instance), C.c_int32, C.c_int32, P]
+        L.vxo_rebuild.restype = C.c_int32
+        L.vxo_build_mip.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, P]
+        L.vxo_get_volume_at.argtypes = [C.POINTER(Occ), C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.vxo_get_volume_at.restype = C.c_int32
+        L.vxo_dda.argtypes = [C.POINTER(Occ), f32p, f32p, C.c_float, C.POINTER(RayResult)]
+        L.vxo_dda.restype = C.c_int32
+        L.vxo_sparse.argtypes = [C.POINTER(Occ), f32p, f32p, C.c_float, C.POINTER(RayResult)]
+        L.vxo_sparse.restype = C.c_float
+        L.vxo_supersparse.argtypes = [C.POINTER(Occ), f32p, f32p, C.c_float, C.POINTER(RayResult)]
+        L.vxo_supersparse.restype = C.c_float
+        L.vxo_trace_rays.argtypes = [C.POINTER(Occ), P, C.c_int32, C.c_int32, P, C.c_int32]
+        L.vxo_clip_to_aabb.argtypes = [f32p, f32p, f32p, f32p]
+        L.vxo_intersect_volume.argtypes = [C.POINTER(P), C.POINTER(C.c_int32), f32p, f32p, f32p, f32p, f32p,
+                                           C.c_int32, f32p, f32p, C.POINTER(C.c_uint8)]
+        L.vxo_intersect_volume.restype = C.c_int32
+        L.vxo_primary.argtypes = [C.POINTER(P), C.POINTER(C.c_int32), C.POINTER(View), P, P, P, C.c_int32]
+        L.vxo_trace.argtypes = [C.POINTER(Occ), C.POINTER(View), C.POINTER(TraceParams), P, P, P, P,
+                                P, P, P, P, P, C.c_int32, C.c_int32]
+        L.vxo_shade_f32.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, C.c_int32]
+        L.vxo_shade_f64.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, C.c_int32]
+        L.vxo_shade_backward_f64.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, P, P]
+        L.vxo_gen_terrain.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_float, C.c_int32]
+        L.vxo_gen_boxes.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_int32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+# ---------------------------------------------------------------- scalars
+def f2i(x):
+    return lib().vxo_f2i(float(np.float32(x)))
+
+
+def sincos(x):
+    s, c = C.c_float(), C.c_float()
+    lib().vxo_sincos(float(np.float32(x)), C.byref(s), C.byref(c))
+    return np.float32(s.value), np.float32(c.value)
+
+
+def hash32(x):
+    return lib().vxo_hash(int(x) & 0xFFFFFFFF)
+
+
+def rand(pixkey, idx):
+    return np.float32(lib().vxo_rand(pixkey & 0xFFFFFFFF, idx & 0xFFFFFFFF))
+
+
+def pixkey(seed, frame, pixel):
+    return lib().vxo_pixkey(seed & 0xFFFFFFFF, frame, pixel & 0xFFFFFFFF)
+
+
+# ---------------------------------------------------------------- rebuild
+def set_volume_at(vol: np.ndarray, x, y, z, value):
+    sz, sy, sx = vol.shape
+    lib().vxo_set_volume_at(_p(vol), sx, sy, sz, x, y, z, value)
+
+
+def translate(m, v):
+    m = np.ascontiguousarray(m, dtype=np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    v = np.asarray(v, np.float32)
+    lib().vxo_translate(m.ctypes.data_as(C.POINTER(C.c_float)), v.ctypes.data_as(C.POINTER(C.c_float)),
+                        out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def make_instance(model: np.ndarray, prev, cur, pivot=(0, 0, 0), flags=INST_UPDATE):
+    """model: uint8 array shaped (sz, sy, sx) (X fastest). Matrices column-major 16 floats."""
+    inst = Instance()
+    inst.model = model.ctypes.data if model is not None else None
+    if model is not None:
+        inst.sz, inst.sy, inst.sx = model.shape
+    inst.prev = (C.c_float * 16)(*np.asarray(prev, np.float32).reshape(16))
+    inst.cur = (C.c_float * 16)(*np.asarray(cur, np.float32).reshape(16))
+    inst.pivot = _f3(pivot)
+    inst.flags = flags
+    return inst
+
+
+def rebuild(vol: np.ndarray, instances, threshold=16):
+    """vol: uint8 (sz, sy, sx) texel volume, modified in place. Returns regions (n, 6) int64
+    with extents as uint32 values."""
+    sz, sy, sx = vol.shape
+    n = len(instances)
+    arr = (Instance * max(n, 1))(*instances)
+    regions = np.zeros((max(n, 1), 6), np.int32)
+    nr = lib().vxo_rebuild(_p(vol), sx, sy, sz, arr, n, threshold, _p(regions))
+    r = regions[:nr].astype(np.int64)
+    r[:, 3:] &= 0xFFFFFFFF
+    return r
+
+
+def build_mip(parent: np.ndarray):
+    pz, py, px = parent.shape
+    out = np.zeros((max(pz // 2, 1), max(py // 2, 1), max(px // 2, 1)), np.uint8)
+    lib().vxo_build_mip(_p(parent), px, py, pz, _p(out))
+    return out
+
+
+def build_mips(model: np.ndarray):
+    m1 = build_mip(model)
+    m2 = build_mip(m1)
+    return [np.ascontiguousarray(model), m1, m2]
+
+
+# ---------------------------------------------------------------- traversal
+def occ(vol: np.ndarray) -> Occ:
+    sz, sy, sx = vol.shape
+    o = Occ()
+    o.data = vol.ctypes.data
+    o.sx, o.sy, o.sz = sx, sy, sz
+    return o
+
+
+def dda(vol, o, d, maxt):
+    r = RayResult()
+    O = occ(vol)
+    lib().vxo_dda(C.byref(O), _f3(o), _f3(d), float(np.float32(maxt)), C.byref(r))
+    return r
+
+
+def sparse(vol, o, d, dist, super_=False):
+    r = RayResult()
+    O = occ(vol)
+    fn = lib().vxo_supersparse if super_ else lib().vxo_sparse
+    fn(C.byref(O), _f3(o), _f3(d), float(np.float32(dist)), C.byref(r))
+    return r
+
+
+def trace_rays(vol, rays: np.ndarray, march, nthreads=0):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 7)
+    out = np.zeros(rays.shape[0], RAY_RESULT_DTYPE)
+    O = occ(vol)
+    lib().vxo_trace_rays(C.byref(O), _p(rays), rays.shape[0], march, _p(out), nthreads)
+    return out
+
+
+def _mips_args(mips):
+    ptrs = (C.c_void_p * 3)(*[m.ctypes.data for m in mips])
+    d = mips[0].shape
+    dims = (C.c_int32 * 3)(d[2], d[1], d[0])
+    return ptrs, dims
+
+
+def intersect_volume(mips, o, d, cam_local, uv, res, frame):
+    ptrs, dims = _mips_args(mips)
+    hit = (C.c_float * 3)(); nrm = (C.c_float * 3)(); mat = C.c_uint8()
+    f2 = lambda v: (C.c_float * 2)(*[float(x) for x in v])
+    h = lib().vxo_intersect_volume(ptrs, dims, _f3(o), _f3(d), _f3(cam_local), f2(uv), f2(res), frame,
+                                   hit, nrm, C.byref(mat))
+    return h, np.array(hit, np.float32), np.array(nrm, np.float32), mat.value
+
+
+def view_struct(vd: dict) -> View:
+    v = View()
+    for k in ("last_view", "view", "inv_view", "proj", "inv_proj"):
+        setattr(v, k, (C.c_float * 16)(*np.asarray(vd[k], np.float32).reshape(16)))
+    v.res = (C.c_float * 2)(*vd["res"]); v.ires = (C.c_float * 2)(*vd["ires"])
+    v.cam_pos = _f3(vd["cam_pos"]); v.jitter = (C.c_float * 2)(*vd["jitter"])
+    v.frame = vd["frame"]; v.width = vd["width"]; v.height = vd["height"]
+    return v
+
+
+def primary(mips, vd: dict, nthreads=0):
+    ptrs, dims = _mips_args(mips)
+    W, H = vd["width"], vd["height"]
+    pos = np.zeros((H * W, 3), np.float32); nrm = np.zeros((H * W, 3), np.float32)
+    mat = np.zeros(H * W, np.uint8)
+    v = view_struct(vd)
+    lib().vxo_primary(ptrs, dims, C.byref(v), _p(pos), _p(nrm), _p(mat), nthreads)
+    return pos, nrm, mat
+
+
+def trace_params(p: dict) -> TraceParams:
+    t = TraceParams()
+    t.shadow_march = p["shadow_march"]; t.ao_march = p["ao_march"]; t.spec_march = p["spec_march"]
+    t.n_shadow = p["n_shadow"]; t.n_ao = p["n_ao"]; t.n_spec = p["n_spec"]
+    t.shadow_maxt = p["shadow_maxt"]; t.ao_maxt = p["ao_maxt"]; t.spec_maxt = p["spec_maxt"]
+    t.light_dir = _f3(p["light_dir"]); t.seed = p["seed"] & 0xFFFFFFFF; t.frame = p["frame"]
+    t.jitter_shadow = p["jitter_shadow"]
+    return t
+
+
+def trace(vol, vd: dict, params: dict, pos, nrm, mat, palette, pixels=None, nthreads=0):
+    npx = vd["width"] * vd["height"]
+    shadow = np.zeros(npx, np.uint8); ao = np.zeros(npx, np.float32); spec = np.zeros(npx, np.float32)
+    fetches = np.zeros(npx, np.uint32)
+    O = occ(vol); v = view_struct(vd); tp = trace_params(params)
+    palette = np.ascontiguousarray(palette, np.float32)
+    if pixels is not None:
+        pixels = np.ascontiguousarray(pixels, np.int32)
+        pp, npix = _p(pixels), pixels.size
+    else:
+        pp, npix = None, 0
+    lib().vxo_trace(C.byref(O), C.byref(v), C.byref(tp), _p(pos), _p(nrm), _p(mat), _p(palette),
+                    _p(shadow), _p(ao), _p(spec), _p(fetches), pp, npix, nthreads)
+    return shadow, ao, spec, fetches
+
+
+def shade_f32(vd, pos, nrm, mat, shadow, ao, palette, light, ssao=None, nthreads=0):
+    npx = vd["width"] * vd["height"]
+    rgb = np.zeros((npx, 3), np.float32)
+    v = view_struct(vd)
+    light = np.ascontiguousarray(light, np.float32)
+    palette = np.ascontiguousarray(palette, np.float32)
+    lib().vxo_shade_f32(C.byref(v), _p(pos), _p(nrm), _p(mat), _p(shadow), _p(ao),
+                        _p(ssao) if ssao is not None else None, _p(palette), _p(light), _p(rgb), nthreads)
+    return rgb
+
+
+def shade_f64(vd, pos, nrm, mat, shadow, ao, palette, light, ssao=None, nthreads=0):
+    npx = vd["width"] * vd["height"]
+    rgb = np.zeros((npx, 3), np.float64)
+    v = view_struct(vd)
+    light = np.ascontiguousarray(light, np.float64)
+    palette = np.ascontiguousarray(palette, np.float64)
+    lib().vxo_shade_f64(C.byref(v), _p(pos), _p(nrm), _p(mat), _p(shadow), _p(ao),
+                        _p(ssao) if ssao is not None else None, _p(palette), _p(light), _p(rgb), nthreads)
+    return rgb
+
+
+def shade_backward_f64(vd, pos, nrm, mat, shadow, ao, palette, light, dl_drgb, ssao=None):
+    v = view_struct(vd)
+    light = np.ascontiguousarray(light, np.float64)
+    palette = np.ascontiguousarray(palette, np.float64)
+    g = np.ascontiguousarray(dl_drgb, np.float64)
+    dpal = np.zeros((256, 6), np.float64); dlight = np.zeros(6, np.float64)
+    lib().vxo_shade_backward_f64(C.byref(v), _p(pos), _p(nrm), _p(mat), _p(shadow), _p(ao),
+                                 _p(ssao) if ssao is not None else None, _p(palette), _p(light), _p(g),
+                                 _p(dpal), _p(dlight))
+    return dpal, dlight
+
+
+# ---------------------------------------------------------------- scenes
+def gen_terrain(nx, ny, nz, seed, fill, nthreads=0):
+    vol = np.zeros((nz, ny, nx), np.uint8)
+    lib().vxo_gen_terrain(_p(vol), nx, ny, nz, seed & 0xFFFFFFFF, float(fill), nthreads)
+    return vol
+
+
+def gen_boxes(vol, seed, nboxes, emin, emax, grounded=0):
+    nz, ny, nx = vol.shape
+    lib().vxo_gen_boxes(_p(vol), nx, ny, nz, seed & 0xFFFFFFFF, nboxes, emin, emax, grounded)
+    return vol

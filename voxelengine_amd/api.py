@@ -1,0 +1,266 @@
+"""Python host API over the C ABI (device buffers are torch tensors on the HIP device).
+
+Used by bench.py, the tests and smoke(). Every call lands in libvx_amd.so; a
+missing library or device raises instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import check
+
+
+def _stream(stream=None):
+    if stream is None:
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "device buffers must be contiguous HIP tensors"
+    return C.c_void_p(t.data_ptr())
+
+
+def _hp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """vx_ctx: one HIP device."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(L.lib().vx_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            L.lib().vx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model:
+    """vx_model: VoxAsset material volume with 3 mips. Arrays are (sz, sy, sx), X fastest."""
+
+    def __init__(self, ctx: Context, sx: int, sy: int, sz: int):
+        h = C.c_void_p()
+        check(L.lib().vx_model_create(ctx.h, sx, sy, sz, C.byref(h)))
+        self.h, self.ctx = h, ctx
+        d = (C.c_int32 * 3)()
+        check(L.lib().vx_model_dims(h, d))
+        self.dims = (d[0], d[1], d[2])
+
+    def close(self):
+        if self.h:
+            L.lib().vx_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, vol: np.ndarray, stream=None):
+        vol = np.ascontiguousarray(vol, np.uint8)
+        check(L.lib().vx_model_upload(self.h, _hp(vol), vol.nbytes, _stream(stream)))
+
+    def download(self, mip: int = 0, stream=None) -> np.ndarray:
+        sx, sy, sz = (d >> mip for d in self.dims)
+        out = np.empty((sz, sy, sx), np.uint8)
+        check(L.lib().vx_model_download(self.h, mip, _hp(out), out.nbytes, _stream(stream)))
+        return out
+
+    def mip_ptr(self, mip: int = 0) -> int:
+        p = C.c_void_p()
+        check(L.lib().vx_model_mip_ptr(self.h, mip, C.byref(p)))
+        return p.value
+
+    def generate(self, seed, terrain_fill=0.3, nboxes=0, box_min=2, box_max=32, grounded=False, stream=None):
+        d = L.SceneDesc(seed & 0xFFFFFFFF, float(terrain_fill), nboxes, box_min, box_max, int(bool(grounded)))
+        check(L.lib().vx_model_generate(self.h, C.byref(d), _stream(stream)))
+
+    def build_mips(self, stream=None):
+        check(L.lib().vx_model_build_mips(self.h, _stream(stream)))
+
+
+def region_tuple(r: L.Region):
+    return (r.x, r.y, r.z, r.w, r.h, r.d)
+
+
+def make_instance(model_ptr: int, dims, prev, cur, pivot=(0.0, 0.0, 0.0), flags=L.INST_UPDATE) -> L.Instance:
+    inst = L.Instance()
+    inst.model = model_ptr
+    inst.sx, inst.sy, inst.sz = dims
+    inst.prev = L.f16(*np.asarray(prev, np.float32).reshape(16))
+    inst.cur = L.f16(*np.asarray(cur, np.float32).reshape(16))
+    inst.pivot = (C.c_float * 3)(*[float(x) for x in pivot])
+    inst.flags = flags
+    return inst
+
+
+IDENTITY = np.eye(4, dtype=np.float32).reshape(16)
+
+
+class Volume:
+    """vx_volume: ShadowVox occupancy volume (texel dims) with coarse levels."""
+
+    def __init__(self, ctx: Context, sx: int, sy: int, sz: int, nlevels: int = 2):
+        h = C.c_void_p()
+        check(L.lib().vx_volume_create(ctx.h, sx, sy, sz, nlevels, C.byref(h)))
+        self.h, self.ctx = h, ctx
+        self.dims = (sx, sy, sz)
+        self.nlevels = nlevels
+
+    def close(self):
+        if self.h:
+            L.lib().vx_volume_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self, stream=None):
+        check(L.lib().vx_volume_clear(self.h, _stream(stream)))
+
+    def rebuild(self, instances, threshold=16, stream=None):
+        n = len(instances)
+        arr = (L.Instance * max(n, 1))(*instances)
+        dirty = (L.Region * max(n, 1))()
+        nd = C.c_uint32()
+        check(L.lib().vx_volume_rebuild(self.h, arr, n, threshold, dirty, C.byref(nd), _stream(stream)))
+        return [region_tuple(dirty[i]) for i in range(nd.value)]
+
+    def build_levels(self, regions=None, stream=None):
+        if regions:
+            arr = (L.Region * len(regions))(*[L.Region(*r) for r in regions])
+            check(L.lib().vx_volume_build_levels(self.h, arr, len(regions), _stream(stream)))
+        else:
+            check(L.lib().vx_volume_build_levels(self.h, None, 0, _stream(stream)))
+
+    def upload_region(self, host: np.ndarray, regions, stream=None):
+        host = np.ascontiguousarray(host, np.uint8)
+        arr = (L.Region * len(regions))(*[L.Region(*r) for r in regions])
+        check(L.lib().vx_volume_upload_region(self.h, _hp(host), arr, len(regions), _stream(stream)))
+
+    def download(self, stream=None) -> np.ndarray:
+        sx, sy, sz = self.dims
+        out = np.empty((sz, sy, sx), np.uint8)
+        check(L.lib().vx_volume_download(self.h, _hp(out), out.nbytes, _stream(stream)))
+        return out
+
+    def download_level(self, level: int, stream=None) -> np.ndarray:
+        total = C.c_size_t()
+        check(L.lib().vx_volume_download_level(self.h, level, None, 0, C.byref(total), _stream(stream)))
+        out = np.empty(total.value, np.uint64)
+        check(L.lib().vx_volume_download_level(self.h, level, _hp(out), out.size, None, _stream(stream)))
+        return out
+
+
+def view_struct(vd: dict) -> L.View:
+    v = L.View()
+    for k in ("last_view", "view", "inv_view", "proj", "inv_proj"):
+        setattr(v, k, L.f16(*np.asarray(vd[k], np.float32).reshape(16)))
+    v.res = (C.c_float * 2)(*[float(x) for x in vd["res"]])
+    v.ires = (C.c_float * 2)(*[float(x) for x in vd["ires"]])
+    v.cam_pos = (C.c_float * 3)(*[float(x) for x in vd["cam_pos"]])
+    v.jitter = (C.c_float * 2)(*[float(x) for x in vd["jitter"]])
+    v.frame, v.width, v.height = vd["frame"], vd["width"], vd["height"]
+    return v
+
+
+def trace_params(p: dict) -> L.TraceParams:
+    t = L.TraceParams()
+    t.shadow_march, t.ao_march, t.spec_march = p["shadow_march"], p["ao_march"], p["spec_march"]
+    t.n_shadow, t.n_ao, t.n_spec = p["n_shadow"], p["n_ao"], p["n_spec"]
+    t.shadow_maxt, t.ao_maxt, t.spec_maxt = p["shadow_maxt"], p["ao_maxt"], p["spec_maxt"]
+    t.light_dir = (C.c_float * 3)(*[float(x) for x in p["light_dir"]])
+    t.seed = p["seed"] & 0xFFFFFFFF
+    t.frame = p["frame"]
+    t.jitter_shadow = p["jitter_shadow"]
+    return t
+
+
+class GBuffer:
+    def __init__(self, width, height, device="cuda"):
+        n = width * height
+        self.width, self.height = width, height
+        self.pos = torch.zeros((n, 3), dtype=torch.float32, device=device)
+        self.normal = torch.zeros((n, 3), dtype=torch.float32, device=device)
+        self.material = torch.zeros(n, dtype=torch.uint8, device=device)
+
+    def struct(self) -> L.GBuf:
+        return L.GBuf(_ptr(self.pos), _ptr(self.normal), _ptr(self.material))
+
+
+class Visibility:
+    def __init__(self, width, height, fetches=False, device="cuda"):
+        n = width * height
+        self.shadow = torch.zeros(n, dtype=torch.uint8, device=device)
+        self.ao = torch.zeros(n, dtype=torch.float32, device=device)
+        self.spec = torch.zeros(n, dtype=torch.float32, device=device)
+        self.fetches = torch.zeros(n, dtype=torch.int32, device=device) if fetches else None
+
+    def struct(self) -> L.TraceOut:
+        return L.TraceOut(_ptr(self.shadow), _ptr(self.ao), _ptr(self.spec), _ptr(self.fetches))
+
+
+def primary(ctx: Context, model: Model, vd: dict, gbuf: GBuffer, stream=None):
+    v = view_struct(vd)
+    g = gbuf.struct()
+    check(L.lib().vx_primary(ctx.h, model.h, C.byref(v), C.byref(g), _stream(stream)))
+
+
+def trace(ctx: Context, vol: Volume, vd: dict, gbuf: GBuffer, palette: torch.Tensor, params: dict,
+          vis: Visibility, stream=None):
+    v = view_struct(vd)
+    g = gbuf.struct()
+    tp = trace_params(params)
+    o = vis.struct()
+    check(L.lib().vx_trace(ctx.h, vol.h, C.byref(v), C.byref(g), _ptr(palette), C.byref(tp), C.byref(o),
+                           _stream(stream)))
+
+
+RAY_RESULT_DTYPE = np.dtype([("hit", "<i4"), ("term", "<i4"), ("steps", "<i4"), ("fetches", "<i4"),
+                             ("voxel", "<i4", 3), ("t", "<f4"), ("hitpos", "<f4", 3), ("normal", "<f4", 3)])
+
+
+def trace_rays(ctx: Context, vol: Volume, rays: torch.Tensor, march: int, stream=None) -> np.ndarray:
+    """rays: (n, 7) float32 device tensor (origin3, dir3, maxt). Returns host records."""
+    n = rays.shape[0]
+    out = torch.zeros((n, RAY_RESULT_DTYPE.itemsize // 4), dtype=torch.int32, device=rays.device)
+    check(L.lib().vx_trace_rays(ctx.h, vol.h, _ptr(rays), n, march, _ptr(out), _stream(stream)))
+    return out.cpu().numpy().view(RAY_RESULT_DTYPE).reshape(n)
+
+
+def _light(light):
+    return (C.c_float * 6)(*[float(x) for x in np.asarray(light, np.float32).reshape(6)])
+
+
+def shade_forward(ctx, vd, gbuf, vis, palette, light, rgb, ssao=None, stream=None):
+    v = view_struct(vd); g = gbuf.struct(); o = vis.struct()
+    check(L.lib().vx_shade_forward(ctx.h, C.byref(v), C.byref(g), C.byref(o), _ptr(ssao), _ptr(palette),
+                                   _light(light), _ptr(rgb), _stream(stream)))
+
+
+def shade_backward(ctx, vd, gbuf, vis, palette, light, dl_drgb, dpalette, dlight, ssao=None, stream=None):
+    v = view_struct(vd); g = gbuf.struct(); o = vis.struct()
+    check(L.lib().vx_shade_backward(ctx.h, C.byref(v), C.byref(g), C.byref(o), _ptr(ssao), _ptr(palette),
+                                    _light(light), _ptr(dl_drgb), _ptr(dpalette), _ptr(dlight), _stream(stream)))

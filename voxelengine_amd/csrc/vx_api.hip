@@ -1,0 +1,745 @@
+// vx_api.hip -- host layer of the C ABI (include/vx/vx_api.h).
+//
+// Owns device volumes and models, validates arguments, plans the rebuild
+// batches and launches the gfx950 kernels of vx_kernels.hip. No CPU fallback:
+// every pass runs on the HIP device or returns an error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vx_internal.hpp"
+
+using namespace vx;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static vx_status fail(vx_status code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define VX_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            (void)hipGetLastError();                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? VX_ERR_OUT_OF_MEMORY : VX_ERR_DEVICE,     \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,      \
+                        __LINE__);                                                            \
+        }                                                                                     \
+    } while (0)
+
+#define VX_REQUIRE(cond, ...)                                                                 \
+    do {                                                                                      \
+        if (!(cond)) return fail(VX_ERR_INVALID_ARGUMENT, __VA_ARGS__);                       \
+    } while (0)
+
+#define VX_LAUNCHED(name)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess) return fail(VX_ERR_DEVICE, "launch of %s failed: %s", name,     \
+                                          hipGetErrorString(e_));                             \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+struct vx_ctx {
+    int32_t device;
+    long long* grad_acc;     // 256*6+6 fixed-point gradient accumulator
+};
+
+struct vx_model {
+    vx_ctx* ctx;
+    int32_t dims[3];
+    uint8_t* mip[3];
+};
+
+struct vx_volume {
+    vx_ctx* ctx;
+    int32_t sx, sy, sz;
+    int32_t nlevels;
+    int32_t bx, by, bz;
+    int32_t wx[3], wy[3], wz[3];    // level word grids
+    int32_t cx[3], cy[3], cz[3];    // level cell grids
+    uint64_t* cells;
+    uint64_t* lvl[3];
+    size_t ncells;
+};
+
+static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+static vx_status use_device(const vx_ctx* ctx) {
+    VX_HIP(hipSetDevice(ctx->device));
+    return VX_OK;
+}
+
+static OccView occ_view(const vx_volume* v) {
+    OccView V;
+    V.cells = v->cells;
+    for (int l = 0; l < 3; ++l) {
+        V.lvl[l] = v->lvl[l];
+        V.wx[l] = v->wx[l]; V.wy[l] = v->wy[l]; V.wz[l] = v->wz[l];
+    }
+    V.sx = v->sx; V.sy = v->sy; V.sz = v->sz;
+    V.bx = v->bx; V.by = v->by; V.bz = v->bz;
+    V.nlevels = v->nlevels;
+    return V;
+}
+
+static TileMap tile_map(int32_t w, int32_t h) {
+    TileMap t;
+    t.width = w;
+    t.height = h;
+    t.tiles_x = cdiv((uint32_t)w, 16);
+    t.ntiles = t.tiles_x * cdiv((uint32_t)h, 16);
+    t.per_xcd = cdiv(t.ntiles, 8);
+    return t;
+}
+
+static vx_status check_view(const vx_view* view) {
+    VX_REQUIRE(view, "view is NULL");
+    VX_REQUIRE(view->width > 0 && view->height > 0 && view->width <= 16384 && view->height <= 16384,
+               "view size %dx%d outside 1..16384", view->width, view->height);
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int32_t vx_api_version(void) { return VX_API_VERSION; }
+extern "C" const char* vx_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
+    VX_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(VX_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    }
+    VX_REQUIRE(device >= 0 && device < n, "device %d out of range (0..%d)", device, n - 1);
+    VX_HIP(hipSetDevice(device));
+    vx_ctx* c = new (std::nothrow) vx_ctx();
+    if (!c) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    c->device = device;
+    c->grad_acc = nullptr;
+    hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
+    if (me != hipSuccess) {
+        delete c;
+        return fail(VX_ERR_OUT_OF_MEMORY, "hipMalloc of the gradient accumulator failed");
+    }
+    *out = c;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
+    if (!ctx) return VX_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipFree(ctx->grad_acc);
+    delete ctx;
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// models (VoxAsset)
+// ---------------------------------------------------------------------------
+extern "C" vx_status vx_model_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32_t sz, vx_model** out) {
+    VX_REQUIRE(ctx && out, "ctx/out is NULL");
+    VX_REQUIRE(sx > 0 && sy > 0 && sz > 0 && sx <= 4096 && sy <= 4096 && sz <= 4096,
+               "model dims %dx%dx%d outside 1..4096", sx, sy, sz);
+    *out = nullptr;
+    if (vx_status st = use_device(ctx)) return st;
+    vx_model* m = new (std::nothrow) vx_model();
+    if (!m) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    m->ctx = ctx;
+    // VoxAsset.h:27-29: round each dim up to a multiple of 4.
+    m->dims[0] = ((sx - 1) & ~3) + 4;
+    m->dims[1] = ((sy - 1) & ~3) + 4;
+    m->dims[2] = ((sz - 1) & ~3) + 4;
+    for (int k = 0; k < 3; ++k) m->mip[k] = nullptr;
+    for (int k = 0; k < 3; ++k) {
+        size_t n = (size_t)(m->dims[0] >> k) * (m->dims[1] >> k) * (m->dims[2] >> k);
+        hipError_t e = hipMalloc(&m->mip[k], n);
+        if (e == hipSuccess) e = hipMemset(m->mip[k], 0, n);
+        if (e != hipSuccess) {
+            for (int j = 0; j < 3; ++j) (void)hipFree(m->mip[j]);
+            delete m;
+            return fail(VX_ERR_OUT_OF_MEMORY, "model allocation failed: %s", hipGetErrorString(e));
+        }
+    }
+    *out = m;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_destroy(vx_model* m) {
+    if (!m) return VX_OK;
+    (void)hipSetDevice(m->ctx->device);
+    for (int k = 0; k < 3; ++k) (void)hipFree(m->mip[k]);
+    delete m;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_dims(const vx_model* m, int32_t dims[3]) {
+    VX_REQUIRE(m && dims, "model/dims is NULL");
+    for (int k = 0; k < 3; ++k) dims[k] = m->dims[k];
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_mip_ptr(const vx_model* m, int32_t mip, uint8_t** dev_ptr) {
+    VX_REQUIRE(m && dev_ptr, "model/dev_ptr is NULL");
+    VX_REQUIRE(mip >= 0 && mip < 3, "mip %d outside 0..2", mip);
+    *dev_ptr = m->mip[mip];
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_upload(vx_model* m, const uint8_t* host, size_t nbytes, void* stream) {
+    VX_REQUIRE(m && host, "model/host is NULL");
+    size_t n = (size_t)m->dims[0] * m->dims[1] * m->dims[2];
+    VX_REQUIRE(nbytes == n, "upload of %zu bytes, model holds %zu", nbytes, n);
+    if (vx_status st = use_device(m->ctx)) return st;
+    VX_HIP(hipMemcpyAsync(m->mip[0], host, n, hipMemcpyHostToDevice, S(stream)));
+    VX_HIP(hipStreamSynchronize(S(stream)));
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_download(const vx_model* m, int32_t mip, uint8_t* host, size_t nbytes, void* stream) {
+    VX_REQUIRE(m && host, "model/host is NULL");
+    VX_REQUIRE(mip >= 0 && mip < 3, "mip %d outside 0..2", mip);
+    size_t n = (size_t)(m->dims[0] >> mip) * (m->dims[1] >> mip) * (m->dims[2] >> mip);
+    VX_REQUIRE(nbytes == n, "download buffer holds %zu bytes, mip %d has %zu", nbytes, mip, n);
+    if (vx_status st = use_device(m->ctx)) return st;
+    VX_HIP(hipMemcpyAsync(host, m->mip[mip], n, hipMemcpyDeviceToHost, S(stream)));
+    VX_HIP(hipStreamSynchronize(S(stream)));
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_build_mips(vx_model* m, void* stream) {
+    VX_REQUIRE(m, "model is NULL");
+    if (vx_status st = use_device(m->ctx)) return st;
+    for (int k = 1; k < 3; ++k) {
+        int32_t px = m->dims[0] >> (k - 1), py = m->dims[1] >> (k - 1), pz = m->dims[2] >> (k - 1);
+        size_t n = (size_t)(px / 2) * (py / 2) * (pz / 2);
+        hipLaunchKernelGGL(k_build_mip, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), m->mip[k - 1], px, py, pz,
+                           m->mip[k]);
+        VX_LAUNCHED("k_build_mip");
+    }
+    return VX_OK;
+}
+
+extern "C" vx_status vx_model_generate(vx_model* m, const vx_scene_desc* d, void* stream) {
+    VX_REQUIRE(m && d, "model/desc is NULL");
+    VX_REQUIRE(d->nboxes >= 0 && (d->nboxes == 0 || (d->box_min >= 1 && d->box_max >= d->box_min)),
+               "bad box parameters");
+    if (vx_status st = use_device(m->ctx)) return st;
+    const int32_t nx = m->dims[0], ny = m->dims[1], nz = m->dims[2];
+    float* h = nullptr;
+    VX_HIP(hipMalloc(&h, (size_t)nx * nz * sizeof(float)));
+    hipLaunchKernelGGL(k_terrain_heights, dim3(cdiv((size_t)nx * nz, 256)), dim3(256), 0, S(stream), h, nx, ny, nz,
+                       d->seed, d->terrain_fill);
+    size_t nq = (size_t)nx * ny * nz / 4;
+    hipLaunchKernelGGL(k_terrain_fill, dim3(cdiv(nq, 256)), dim3(256), 0, S(stream), m->mip[0], h, nx, ny, nz,
+                       d->seed, d->terrain_fill > 0.0f ? 1 : 0);
+    for (int32_t b0 = 0; b0 < d->nboxes; b0 += 65535) {
+        int32_t nb = std::min(65535, d->nboxes - b0);
+        uint32_t maxvox = (uint32_t)d->box_max * d->box_max * d->box_max;
+        hipLaunchKernelGGL(k_boxes, dim3(std::min<uint32_t>(cdiv(maxvox, 256), 64), nb), dim3(256), 0, S(stream),
+                           m->mip[0], nx, ny, nz, d->seed, b0, d->box_min, d->box_max, d->boxes_grounded);
+    }
+    VX_LAUNCHED("scene generation");
+    VX_HIP(hipStreamSynchronize(S(stream)));
+    VX_HIP(hipFree(h));
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// occupancy volume (ShadowVoxSystem)
+// ---------------------------------------------------------------------------
+extern "C" vx_status vx_volume_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32_t sz, int32_t nlevels,
+                                      vx_volume** out) {
+    VX_REQUIRE(ctx && out, "ctx/out is NULL");
+    VX_REQUIRE(sx > 0 && sy > 0 && sz > 0 && sx <= 4096 && sy <= 4096 && sz <= 4096,
+               "volume texel dims %dx%dx%d outside 1..4096", sx, sy, sz);
+    VX_REQUIRE(nlevels >= 0 && nlevels <= 3, "nlevels %d outside 0..3", nlevels);
+    *out = nullptr;
+    if (vx_status st = use_device(ctx)) return st;
+    vx_volume* v = new (std::nothrow) vx_volume();
+    if (!v) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    v->ctx = ctx;
+    v->sx = sx; v->sy = sy; v->sz = sz;
+    v->nlevels = nlevels;
+    // One extra brick per axis keeps texel index s (fine 2s, 2s+1) inside the grid.
+    v->bx = sx / 4 + 1; v->by = sy / 4 + 1; v->bz = sz / 4 + 1;
+    v->ncells = (size_t)v->bx * v->by * v->bz * 8;
+    v->cells = nullptr;
+    for (int l = 0; l < 3; ++l) {
+        v->lvl[l] = nullptr;
+        int32_t shift = 3 * l;   // level l+1 cell = brick >> 3l
+        v->cx[l] = (int32_t)cdiv(v->bx, 1u << shift);
+        v->cy[l] = (int32_t)cdiv(v->by, 1u << shift);
+        v->cz[l] = (int32_t)cdiv(v->bz, 1u << shift);
+        v->wx[l] = (int32_t)cdiv(v->cx[l], 4);
+        v->wy[l] = (int32_t)cdiv(v->cy[l], 4);
+        v->wz[l] = (int32_t)cdiv(v->cz[l], 4);
+    }
+    hipError_t e = hipMalloc(&v->cells, v->ncells * 8);
+    if (e == hipSuccess) e = hipMemset(v->cells, 0, v->ncells * 8);
+    for (int l = 0; l < nlevels && e == hipSuccess; ++l) {
+        size_t nw = (size_t)v->wx[l] * v->wy[l] * v->wz[l];
+        e = hipMalloc(&v->lvl[l], nw * 8);
+        if (e == hipSuccess) e = hipMemset(v->lvl[l], 0, nw * 8);
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(v->cells);
+        for (int l = 0; l < 3; ++l) (void)hipFree(v->lvl[l]);
+        delete v;
+        return fail(VX_ERR_OUT_OF_MEMORY, "volume allocation failed: %s", hipGetErrorString(e));
+    }
+    *out = v;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_destroy(vx_volume* v) {
+    if (!v) return VX_OK;
+    (void)hipSetDevice(v->ctx->device);
+    (void)hipFree(v->cells);
+    for (int l = 0; l < 3; ++l) (void)hipFree(v->lvl[l]);
+    delete v;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_dims(const vx_volume* v, int32_t dims[3], int32_t* nlevels) {
+    VX_REQUIRE(v && dims, "volume/dims is NULL");
+    dims[0] = v->sx; dims[1] = v->sy; dims[2] = v->sz;
+    if (nlevels) *nlevels = v->nlevels;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_clear(vx_volume* v, void* stream) {
+    VX_REQUIRE(v, "volume is NULL");
+    if (vx_status st = use_device(v->ctx)) return st;
+    hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(v->ncells, 256)), dim3(256), 0, S(stream), v->cells, v->ncells, 0ull);
+    for (int l = 0; l < v->nlevels; ++l) {
+        size_t nw = (size_t)v->wx[l] * v->wy[l] * v->wz[l];
+        hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nw, 256)), dim3(256), 0, S(stream), v->lvl[l], nw, 0ull);
+    }
+    VX_LAUNCHED("k_fill_u64");
+    return VX_OK;
+}
+
+static vx_status check_region(const vx_volume* v, const vx_region& r) {
+    VX_REQUIRE(r.x >= 0 && r.y >= 0 && r.z >= 0 && r.w > 0 && r.h > 0 && r.d > 0 &&
+                   (int64_t)r.x + r.w <= v->sx && (int64_t)r.y + r.h <= v->sy && (int64_t)r.z + r.d <= v->sz,
+               "region (%d,%d,%d)+(%u,%u,%u) outside the %dx%dx%d volume", r.x, r.y, r.z, r.w, r.h, r.d, v->sx,
+               v->sy, v->sz);
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_upload_region(vx_volume* v, const uint8_t* host, const vx_region* regions,
+                                             uint32_t nregions, void* stream) {
+    VX_REQUIRE(v && host, "volume/host is NULL");
+    VX_REQUIRE(nregions == 0 || regions, "regions is NULL");
+    for (uint32_t i = 0; i < nregions; ++i)
+        if (vx_status st = check_region(v, regions[i])) return st;
+    if (nregions == 0) return VX_OK;
+    if (vx_status st = use_device(v->ctx)) return st;
+    size_t nb = (size_t)v->sx * v->sy * v->sz;
+    uint8_t* stage = nullptr;
+    VX_HIP(hipMalloc(&stage, nb));
+    hipError_t e = hipMemcpyAsync(stage, host, nb, hipMemcpyHostToDevice, S(stream));
+    OccView V = occ_view(v);
+    for (uint32_t i = 0; i < nregions && e == hipSuccess; ++i) {
+        const vx_region& r = regions[i];
+        size_t n = (size_t)r.w * r.h * r.d;
+        hipLaunchKernelGGL(k_bytes_to_cells, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), v->cells, stage, V, r.x,
+                           r.y, r.z, (int32_t)r.w, (int32_t)r.h, (int32_t)r.d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "region upload failed: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_download(const vx_volume* v, uint8_t* host, size_t nbytes, void* stream) {
+    VX_REQUIRE(v && host, "volume/host is NULL");
+    size_t nb = (size_t)v->sx * v->sy * v->sz;
+    VX_REQUIRE(nbytes == nb, "download buffer holds %zu bytes, volume has %zu", nbytes, nb);
+    if (vx_status st = use_device(v->ctx)) return st;
+    uint8_t* stage = nullptr;
+    VX_HIP(hipMalloc(&stage, nb));
+    hipLaunchKernelGGL(k_cells_to_bytes, dim3(cdiv(nb, 256)), dim3(256), 0, S(stream), stage, v->cells, occ_view(v));
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, stage, nb, hipMemcpyDeviceToHost, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "volume download failed: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+// glm::translate(m, v) (Vendor/glm/ext/matrix_transform.inl:10-15):
+// Result[3] = ((m[0]*v[0] + m[1]*v[1]) + m[2]*v[2]) + m[3].
+static void translate(const float m[16], const float v[3], float out[16]) {
+    std::memcpy(out, m, 16 * sizeof(float));
+    for (int r = 0; r < 4; ++r) {
+        float a = m[0 * 4 + r] * v[0];
+        float b = m[1 * 4 + r] * v[1];
+        float c = m[2 * 4 + r] * v[2];
+        out[3 * 4 + r] = ((a + b) + c) + m[3 * 4 + r];
+    }
+}
+
+struct PlannedOp {
+    FootprintOp op;
+    int64_t lo[3], hi[3];   // conservative fine-voxel footprint box
+};
+
+static PlannedOp make_op(const vx_instance& in, const float m[16], int32_t value, int32_t slot) {
+    PlannedOp p;
+    FootprintOp& o = p.op;
+    o.model = in.model;
+    o.sx = in.sx; o.sy = in.sy; o.sz = in.sz;
+    o.value = value;
+    o.inst = slot;
+    for (int k = 0; k < 3; ++k) {
+        o.o[k] = m[12 + k];
+        o.dx[k] = m[0 + k] * 0.1f;
+        o.dy[k] = m[4 + k] * 0.1f;
+        o.dz[k] = m[8 + k] * 0.1f;
+    }
+    // Corners of the voxel index box mapped exactly, widened by two fine
+    // voxels for fp32 rounding and truncation.
+    for (int k = 0; k < 3; ++k) {
+        double mn = 1e300, mx = -1e300;
+        for (int c = 0; c < 8; ++c) {
+            double x = (c & 1) ? in.sx - 1 : 0, y = (c & 2) ? in.sy - 1 : 0, z = (c & 4) ? in.sz - 1 : 0;
+            double w = ((double)o.o[k] + x * o.dx[k] + y * o.dy[k] + z * o.dz[k]) * 10.0;
+            mn = std::min(mn, w);
+            mx = std::max(mx, w);
+        }
+        double span = std::fabs((double)o.dx[k]) * in.sx + std::fabs((double)o.dy[k]) * in.sy +
+                      std::fabs((double)o.dz[k]) * in.sz + std::fabs((double)o.o[k]);
+        double slack = 2.0 + span * 10.0 * 1e-6;
+        p.lo[k] = (int64_t)std::floor(std::max(mn - slack, -9.0e18));
+        p.hi[k] = (int64_t)std::ceil(std::min(mx + slack, 9.0e18));
+    }
+    return p;
+}
+
+static bool disjoint(const PlannedOp& a, const PlannedOp& b) {
+    for (int k = 0; k < 3; ++k)
+        if (a.hi[k] < b.lo[k] || b.hi[k] < a.lo[k]) return true;
+    return false;
+}
+
+extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, uint32_t n, int32_t thr,
+                                       vx_region* dirty, uint32_t* ndirty, void* stream) {
+    VX_REQUIRE(v, "volume is NULL");
+    VX_REQUIRE(n == 0 || inst, "instances is NULL");
+    VX_REQUIRE(dirty || n == 0, "dirty is NULL");
+    VX_REQUIRE(ndirty, "ndirty is NULL");
+    VX_REQUIRE(thr >= 0 && thr <= 256, "occupancy threshold %d outside 0..256", thr);
+    *ndirty = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const vx_instance& in = inst[i];
+        VX_REQUIRE(in.flags == VX_INST_UPDATE || in.flags == VX_INST_DESTROY, "instance %u: bad flags %d", i,
+                   in.flags);
+        if (in.model)
+            VX_REQUIRE(in.sx > 0 && in.sy > 0 && in.sz > 0 && (int64_t)in.sx * in.sy * in.sz < (1ll << 40),
+                       "instance %u: bad model dims", i);
+    }
+    if (n == 0) return VX_OK;
+    if (vx_status st = use_device(v->ctx)) return st;
+
+    // Operation list in reference order, then greedy batching: an operation
+    // joins the latest compatible batch it can be hoisted into. It may move
+    // past a batch of the other value only when its footprint box is disjoint
+    // from every operation in that batch; same-value operations commute.
+    std::vector<PlannedOp> ops;
+    for (uint32_t i = 0; i < n; ++i) {
+        const vx_instance& in = inst[i];
+        if (!in.model) continue;   // ShadowVoxSystem.cpp:120 (invalid Vox asset)
+        if (in.flags == VX_INST_DESTROY) {
+            ops.push_back(make_op(in, in.cur, 0, (int32_t)i));
+        } else {
+            float np[3] = {-in.pivot[0], -in.pivot[1], -in.pivot[2]};
+            float m[16];
+            translate(in.prev, np, m);
+            ops.push_back(make_op(in, m, 0, (int32_t)i));
+            translate(in.cur, np, m);
+            ops.push_back(make_op(in, m, 1, (int32_t)i));
+        }
+    }
+    std::vector<std::vector<int>> batches;
+    std::vector<int32_t> batch_value;
+    for (int j = 0; j < (int)ops.size(); ++j) {
+        int target = -1;
+        for (int b = (int)batches.size() - 1; b >= 0; --b) {
+            if (batch_value[b] == ops[j].op.value) { target = b; continue; }
+            bool ok = true;
+            for (int q : batches[b])
+                if (!disjoint(ops[q], ops[j])) { ok = false; break; }
+            if (!ok) break;
+        }
+        if (target < 0) {
+            batches.push_back({j});
+            batch_value.push_back(ops[j].op.value);
+        } else {
+            batches[target].push_back(j);
+        }
+    }
+
+    // Per-instance AABB slots start at (startmin, startmax) (ShadowVoxSystem.cpp:128-132).
+    std::vector<int32_t> aabb_h((size_t)n * 6);
+    for (uint32_t i = 0; i < n; ++i) {
+        aabb_h[6 * i + 0] = v->sx - 1; aabb_h[6 * i + 1] = v->sy - 1; aabb_h[6 * i + 2] = v->sz - 1;
+        aabb_h[6 * i + 3] = 0; aabb_h[6 * i + 4] = 0; aabb_h[6 * i + 5] = 0;
+    }
+    int32_t* aabb_d = nullptr;
+    FootprintOp* ops_d = nullptr;
+    VX_HIP(hipMalloc(&aabb_d, aabb_h.size() * sizeof(int32_t)));
+    hipError_t e = hipMalloc(&ops_d, ops.size() * sizeof(FootprintOp));
+    std::vector<FootprintOp> flat;
+    std::vector<size_t> first;
+    for (auto& b : batches) {
+        first.push_back(flat.size());
+        for (int q : b) flat.push_back(ops[q].op);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(aabb_d, aabb_h.data(), aabb_h.size() * sizeof(int32_t), hipMemcpyHostToDevice, S(stream));
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(ops_d, flat.data(), flat.size() * sizeof(FootprintOp), hipMemcpyHostToDevice, S(stream));
+    OccView V = occ_view(v);
+    for (size_t b = 0; b < batches.size() && e == hipSuccess; ++b) {
+        size_t maxvox = 0;
+        for (int q : batches[b]) maxvox = std::max(maxvox, (size_t)ops[q].op.sx * ops[q].op.sy * ops[q].op.sz);
+        uint32_t gx = std::min<uint32_t>(cdiv(maxvox, 256), 4096);
+        for (size_t c0 = 0; c0 < batches[b].size() && e == hipSuccess; c0 += 65535) {
+            uint32_t cnt = (uint32_t)std::min<size_t>(65535, batches[b].size() - c0);
+            hipLaunchKernelGGL(k_footprint, dim3(gx, cnt), dim3(256), 0, S(stream), v->cells, V,
+                               ops_d + first[b] + c0, aabb_d, thr);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(aabb_h.data(), aabb_d, aabb_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    (void)hipFree(aabb_d);
+    (void)hipFree(ops_d);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "rebuild failed: %s", hipGetErrorString(e));
+
+    // Dirty regions, ShadowVoxSystem.cpp:180-190.
+    uint32_t nd = 0;
+    const int32_t smin[3] = {v->sx - 1, v->sy - 1, v->sz - 1};
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!inst[i].model) continue;
+        const int32_t* a = &aabb_h[6 * i];
+        if (a[3] == 0 && a[4] == 0 && a[5] == 0) continue;
+        int32_t mn[3], mx[3];
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = std::max(a[k] / 2, 0);
+            mx[k] = std::min(a[3 + k] / 2, smin[k]);
+        }
+        vx_region& r = dirty[nd++];
+        r.x = mn[0]; r.y = mn[1]; r.z = mn[2];
+        r.w = (uint32_t)(mx[0] - mn[0] + 1);
+        r.h = (uint32_t)(mx[1] - mn[1] + 1);
+        r.d = (uint32_t)(mx[2] - mn[2] + 1);
+    }
+    *ndirty = nd;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_build_levels(vx_volume* v, const vx_region* regions, uint32_t nregions, void* stream) {
+    VX_REQUIRE(v, "volume is NULL");
+    VX_REQUIRE(nregions == 0 || regions, "regions is NULL");
+    if (v->nlevels == 0) return VX_OK;
+    if (vx_status st = use_device(v->ctx)) return st;
+    // Brick ranges to refresh ([lo, hi] inclusive), whole grid when no regions.
+    std::vector<std::array<int32_t, 6>> ranges;
+    if (nregions == 0) {
+        ranges.push_back({0, 0, 0, v->bx - 1, v->by - 1, v->bz - 1});
+    } else {
+        for (uint32_t i = 0; i < nregions; ++i) {
+            const vx_region& r = regions[i];
+            if (r.w == 0 || r.h == 0 || r.d == 0) continue;
+            int64_t lo[3] = {r.x, r.y, r.z};
+            int64_t hi[3] = {(int64_t)r.x + r.w - 1, (int64_t)r.y + r.h - 1, (int64_t)r.z + r.d - 1};
+            const int32_t tdim[3] = {v->sx, v->sy, v->sz};
+            bool empty = false;
+            std::array<int32_t, 6> br;
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::max<int64_t>(lo[k], 0);
+                hi[k] = std::min<int64_t>(hi[k], tdim[k] - 1);
+                if (hi[k] < lo[k]) empty = true;
+                br[k] = (int32_t)(lo[k] >> 2);
+                br[3 + k] = (int32_t)(hi[k] >> 2);
+            }
+            if (!empty) ranges.push_back(br);
+        }
+    }
+    OccView V = occ_view(v);
+    for (auto& br : ranges) {
+        for (int l = 0; l < v->nlevels; ++l) {
+            // level l+1 words covering the brick range: word = brick >> (3l + 2)
+            int32_t sh = 3 * l + 2;
+            int32_t w0[3], w1[3];
+            for (int k = 0; k < 3; ++k) { w0[k] = br[k] >> sh; w1[k] = br[3 + k] >> sh; }
+            int32_t nw[3] = {w1[0] - w0[0] + 1, w1[1] - w0[1] + 1, w1[2] - w0[2] + 1};
+            size_t nthreads = (size_t)nw[0] * nw[1] * nw[2] * 64;
+            if (l == 0) {
+                hipLaunchKernelGGL(k_level1, dim3(cdiv(nthreads, 256)), dim3(256), 0, S(stream), v->lvl[0], v->cells,
+                                   V, w0[0], w0[1], w0[2], nw[0], nw[1], nw[2]);
+            } else {
+                hipLaunchKernelGGL(k_level_up, dim3(cdiv(nthreads, 256)), dim3(256), 0, S(stream), v->lvl[l],
+                                   v->lvl[l - 1], v->wx[l - 1], v->wy[l - 1], v->wz[l - 1], v->wx[l], v->wy[l],
+                                   v->wz[l], w0[0], w0[1], w0[2], nw[0], nw[1], nw[2]);
+            }
+            VX_LAUNCHED("level build");
+        }
+    }
+    return VX_OK;
+}
+
+extern "C" vx_status vx_volume_download_level(const vx_volume* v, int32_t level, uint64_t* host, size_t nwords,
+                                              size_t* words_total, void* stream) {
+    VX_REQUIRE(v, "volume is NULL");
+    VX_REQUIRE(level >= 1 && level <= v->nlevels, "level %d outside 1..%d", level, v->nlevels);
+    size_t nw = (size_t)v->wx[level - 1] * v->wy[level - 1] * v->wz[level - 1];
+    if (words_total) *words_total = nw;
+    if (!host) return VX_OK;
+    VX_REQUIRE(nwords >= nw, "buffer holds %zu words, level has %zu", nwords, nw);
+    if (vx_status st = use_device(v->ctx)) return st;
+    VX_HIP(hipMemcpyAsync(host, v->lvl[level - 1], nw * 8, hipMemcpyDeviceToHost, S(stream)));
+    VX_HIP(hipStreamSynchronize(S(stream)));
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// passes
+// ---------------------------------------------------------------------------
+extern "C" vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_view* view, const vx_gbuf* g,
+                                void* stream) {
+    VX_REQUIRE(ctx && model && g, "ctx/model/gbuf is NULL");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g->pos && g->normal && g->material, "gbuf has a NULL pointer");
+    if (vx_status st = use_device(ctx)) return st;
+    PrimaryArgs A;
+    A.view = *view;
+    A.tiles = tile_map(view->width, view->height);
+    for (int k = 0; k < 3; ++k) { A.mips[k] = model->mip[k]; A.dims[k] = model->dims[k]; }
+    A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
+    hipLaunchKernelGGL(k_primary, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_primary");
+    return VX_OK;
+}
+
+static vx_status check_params(const vx_trace_params* p) {
+    VX_REQUIRE(p, "params is NULL");
+    auto okm = [](int32_t m) { return m == VX_MARCH_DDA || m == VX_MARCH_SPARSE || m == VX_MARCH_SUPERSPARSE; };
+    VX_REQUIRE(okm(p->shadow_march) && okm(p->ao_march) && okm(p->spec_march), "unknown marcher");
+    VX_REQUIRE(p->n_shadow >= 0 && p->n_shadow <= 1 && p->n_spec >= 0 && p->n_spec <= 1 && p->n_ao >= 0 &&
+                   p->n_ao <= 64,
+               "ray counts outside shadow 0..1, ao 0..64, spec 0..1");
+    VX_REQUIRE(p->shadow_maxt > 0.0f && p->ao_maxt > 0.0f && p->spec_maxt > 0.0f, "maxt must be positive");
+    return VX_OK;
+}
+
+extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                              const float* palette, const vx_trace_params* params, const vx_trace_out* out,
+                              void* stream) {
+    VX_REQUIRE(ctx && vol && g && out && palette, "ctx/volume/gbuf/out/palette is NULL");
+    if (vx_status st = check_view(view)) return st;
+    if (vx_status st = check_params(params)) return st;
+    VX_REQUIRE(g->pos && g->normal && g->material, "gbuf has a NULL pointer");
+    VX_REQUIRE(out->shadow && out->ao && out->spec, "trace output has a NULL pointer");
+    if (vx_status st = use_device(ctx)) return st;
+    TraceArgs A;
+    A.view = *view;
+    A.params = *params;
+    A.tiles = tile_map(view->width, view->height);
+    A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
+    A.palette = palette;
+    A.shadow = out->shadow; A.ao = out->ao; A.spec = out->spec; A.fetches = out->fetches;
+    hipLaunchKernelGGL(k_trace, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
+    VX_LAUNCHED("k_trace");
+    return VX_OK;
+}
+
+extern "C" vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const float* rays, uint32_t n, int32_t march,
+                                   vx_ray_result* out, void* stream) {
+    VX_REQUIRE(ctx && vol, "ctx/volume is NULL");
+    VX_REQUIRE(n == 0 || (rays && out), "rays/out is NULL");
+    VX_REQUIRE(march == VX_MARCH_DDA || march == VX_MARCH_SPARSE || march == VX_MARCH_SUPERSPARSE, "unknown marcher %d",
+               march);
+    if (n == 0) return VX_OK;
+    if (vx_status st = use_device(ctx)) return st;
+    hipLaunchKernelGGL(k_trace_rays, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), occ_view(vol), rays, n, march, out);
+    VX_LAUNCHED("k_trace_rays");
+    return VX_OK;
+}
+
+static vx_status shade_args(const vx_view* view, const vx_gbuf* g, const vx_trace_out* vis, const float* ssao,
+                            const float* palette, const float light[6], ShadeArgs& A) {
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g && vis && palette && light, "gbuf/vis/palette/light is NULL");
+    VX_REQUIRE(g->pos && g->normal && g->material && vis->shadow && vis->ao, "NULL buffer in gbuf/vis");
+    A.view = *view;
+    A.npx = (size_t)view->width * view->height;
+    A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
+    A.shadow = vis->shadow; A.ao = vis->ao; A.ssao = ssao;
+    A.palette = palette;
+    for (int k = 0; k < 6; ++k) A.light[k] = light[k];
+    A.rgb = nullptr;
+    A.dl_drgb = nullptr;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_shade_forward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* g, const vx_trace_out* vis,
+                                      const float* ssao, const float* palette, const float light[6], float* rgb,
+                                      void* stream) {
+    VX_REQUIRE(ctx && rgb, "ctx/rgb is NULL");
+    ShadeArgs A;
+    if (vx_status st = shade_args(view, g, vis, ssao, palette, light, A)) return st;
+    if (vx_status st = use_device(ctx)) return st;
+    A.rgb = rgb;
+    hipLaunchKernelGGL(k_shade_fwd, dim3(cdiv(A.npx, 256)), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_shade_fwd");
+    return VX_OK;
+}
+
+extern "C" vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* g, const vx_trace_out* vis,
+                                       const float* ssao, const float* palette, const float light[6],
+                                       const float* dl_drgb, float* dpalette, float* dlight, void* stream) {
+    VX_REQUIRE(ctx && dl_drgb && dpalette && dlight, "ctx/dl_drgb/dpalette/dlight is NULL");
+    ShadeArgs A;
+    if (vx_status st = shade_args(view, g, vis, ssao, palette, light, A)) return st;
+    if (vx_status st = use_device(ctx)) return st;
+    A.dl_drgb = dl_drgb;
+    const int nacc = 256 * 6 + 6;
+    hipLaunchKernelGGL(k_zero_i64, dim3(cdiv(nacc, 256)), dim3(256), 0, S(stream), ctx->grad_acc, nacc);
+    uint32_t grid = std::min<uint32_t>(cdiv(A.npx, 256), 1024);
+    hipLaunchKernelGGL(k_shade_bwd, dim3(grid), dim3(256), 0, S(stream), A, ctx->grad_acc);
+    hipLaunchKernelGGL(k_shade_bwd_finish, dim3(cdiv(nacc, 256)), dim3(256), 0, S(stream), ctx->grad_acc, dpalette,
+                       dlight);
+    VX_LAUNCHED("k_shade_bwd");
+    return VX_OK;
+}

@@ -1,0 +1,149 @@
+// vx_device.hpp -- gfx950 device-side scalar semantics and occupancy access.
+//
+// Every float operation here is a single IEEE op (+,-,*,/,sqrt) in a fixed
+// order; the library is compiled with -ffp-contract=off, correctly rounded
+// fp32 divide/sqrt and fp32 denormals preserved, so results are identical
+// bit for bit to a strict IEEE host evaluation of the same expressions.
+// GLSL builtins follow the conventions pinned for this path:
+//   sign(+-0) = +0, sign(NaN) = 0;  step(e,x) = x < e ? 0 : 1;
+//   min(x,y) = y < x ? y : x;  max(x,y) = x < y ? y : x;
+//   mod(x,y) = x - y*floor(x/y);  ivec3() truncates toward zero, saturates
+//   out-of-range values, NaN -> 0;  out-of-range texel fetches read 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vx {
+
+#define VXD __device__ __forceinline__
+
+VXD int32_t f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x < -2147483648.0f) return INT32_MIN;
+    return (int32_t)x;
+}
+VXD float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+VXD float gstep(float e, float x) { return x < e ? 0.0f : 1.0f; }
+VXD float gmin(float x, float y) { return y < x ? y : x; }
+VXD float gmax(float x, float y) { return x < y ? y : x; }
+VXD float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+VXD float gmod(float x, float y) { return x - y * floorf(x / y); }
+
+struct f3 { float x, y, z; };
+VXD f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+VXD float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+VXD f3 normalize3(f3 v) {
+    float l = sqrtf(dot3(v, v));
+    return mk3(v.x / l, v.y / l, v.z / l);
+}
+VXD f3 load3(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+// Column-major mat4 * vec4: r_i = ((m0i*x + m1i*y) + m2i*z) + m3i*w.
+VXD void mat_vec(const float* m, const float v[4], float out[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        out[r] = ((m[0 + r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r] * v[3];
+}
+
+// sin/cos: Cody-Waite reduction by pi/2 (three-part constant) and the Cephes
+// single-precision polynomials on [-pi/4, pi/4]; only +,-,*.
+VXD void sincos_vx(float x, float* s, float* c) {
+    const float two_over_pi = 0.636619772367581343f;
+    const float c1 = 1.5703125f;
+    const float c2 = 4.837512969970703125e-4f;
+    const float c3 = 7.54978995489188216e-8f;
+    float kf = floorf(x * two_over_pi + 0.5f);
+    int32_t k = f2i(kf);
+    float r = ((x - kf * c1) - kf * c2) - kf * c3;
+    float z = r * r;
+    float sp = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * r + r;
+    float cp = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z
+               - 0.5f * z + 1.0f;
+    switch (k & 3) {
+        case 0: *s = sp; *c = cp; break;
+        case 1: *s = cp; *c = -sp; break;
+        case 2: *s = -sp; *c = -cp; break;
+        default: *s = -cp; *c = sp; break;
+    }
+}
+
+// Counter-hash RNG (lowbias32) replacing the blue-noise texture.
+VXD uint32_t hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+VXD uint32_t pixkey(uint32_t seed, int32_t frame, uint32_t pixel) {
+    uint32_t key = hash32(seed ^ hash32((uint32_t)frame + 0x9e3779b9u));
+    return hash32(key ^ hash32(pixel));
+}
+VXD float rand01(uint32_t pk, uint32_t idx) {
+    return (float)(hash32(pk + idx * 0x9e3779b9u) >> 8) * (1.0f / 16777216.0f);
+}
+
+// cosineSampleHemisphere (LightAmbient.frag:81-87).
+VXD f3 cosine_hemisphere(float u1, float u2) {
+    float r = sqrtf(u1);
+    float theta = 6.283f * u2;
+    float sn, cs;
+    sincos_vx(theta, &sn, &cs);
+    return mk3(r * cs, r * sn, sqrtf(gmax(0.0f, 1.0f - u1)));
+}
+
+// Tangent frame of calculateAmbientIrradiance (LightAmbient.frag:112-113).
+VXD void tangent_frame(f3 n, f3& t, f3& b) {
+    if (fabsf(n.z) > 0.5f) t = mk3(0.0f, -n.z, n.y);
+    else t = mk3(-n.y, n.x, 0.0f);
+    b = mk3(n.y * t.z - n.z * t.y, n.z * t.x - n.x * t.z, n.x * t.y - n.y * t.x);
+}
+
+// ---------------------------------------------------------------------------
+// Occupancy volume in brick layout.
+//   fine voxel f -> texel f>>1 -> cell f>>2 (u64 = 2x2x2 texels) -> brick f>>3
+//   (64 B = 2x2x2 cells). Bricks X-fastest; cell c within a brick and texel t
+//   within a cell are (x&1)|(y&1)<<1|(z&1)<<2 of the respective coordinates;
+//   bit b within the texel byte is the reference's (ShadowVoxSystem.cpp:85).
+//   Brick grid is padded by one so texel index s (the first out-of-range one,
+//   reachable by the DDA bound 2*s+1) stays inside and reads 0.
+// Coarse level k (1..3): 1 bit per (8^k)^3 fine voxels, u64 words of 4x4x4
+// bits, bit (cx&3)|(cy&3)<<2|(cz&3)<<4, words X-fastest.
+// ---------------------------------------------------------------------------
+struct OccView {
+    const uint64_t* cells;
+    const uint64_t* lvl[3];
+    int32_t sx, sy, sz;        // texel dims
+    int32_t bx, by, bz;        // brick grid (padded)
+    int32_t wx[3], wy[3], wz[3];  // level word grids
+    int32_t nlevels;
+};
+
+VXD uint32_t sub3(int32_t x, int32_t y, int32_t z) {
+    return (uint32_t)((x & 1) | ((y & 1) << 1) | ((z & 1) << 2));
+}
+VXD size_t brick_index(const OccView& V, int32_t bx, int32_t by, int32_t bz) {
+    return ((size_t)bz * V.by + by) * V.bx + bx;
+}
+// Cell word index of fine voxel f (f inside the padded brick grid).
+VXD size_t cell_index(const OccView& V, int32_t fx, int32_t fy, int32_t fz) {
+    return brick_index(V, fx >> 3, fy >> 3, fz >> 3) * 8 + sub3(fx >> 2, fy >> 2, fz >> 2);
+}
+// Bit of fine voxel f inside its cell word.
+VXD uint32_t cell_bit(int32_t fx, int32_t fy, int32_t fz) {
+    return sub3(fx >> 1, fy >> 1, fz >> 1) * 8u + sub3(fx, fy, fz);
+}
+// Reference byte of texel t (0 when outside [0, s)), texelFetch semantics.
+VXD uint32_t texel_byte(const OccView& V, int32_t tx, int32_t ty, int32_t tz) {
+    if ((uint32_t)tx >= (uint32_t)V.sx || (uint32_t)ty >= (uint32_t)V.sy || (uint32_t)tz >= (uint32_t)V.sz) return 0u;
+    uint64_t w = V.cells[brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1)];
+    return (uint32_t)(w >> (sub3(tx, ty, tz) * 8u)) & 0xffu;
+}
+VXD bool level_bit(const OccView& V, int32_t lvl, int32_t cx, int32_t cy, int32_t cz) {
+    size_t wi = ((size_t)(cz >> 2) * V.wy[lvl] + (cy >> 2)) * V.wx[lvl] + (cx >> 2);
+    uint32_t b = (uint32_t)((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
+    return (V.lvl[lvl][wi] >> b) & 1ull;
+}
+
+}  // namespace vx

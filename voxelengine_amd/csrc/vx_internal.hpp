@@ -1,0 +1,88 @@
+// vx_internal.hpp -- structures shared by the host layer and the kernels.
+#pragma once
+
+#include "vx_device.hpp"
+#include "../../include/vx/vx_api.h"
+
+namespace vx {
+
+// One footprint loop of the rebuild (ShadowVoxSystem.cpp:137-177 / 22-40):
+// origin and axis steps of M' = translate(M, -pivot), already scaled by 0.1.
+struct FootprintOp {
+    const uint8_t* model;
+    int32_t sx, sy, sz;
+    int32_t value;     // 0 erase, 1 write
+    int32_t inst;      // instance slot for the dirty AABB
+    float o[3], dx[3], dy[3], dz[3];
+};
+
+// Block -> pixel tile mapping (16x16 pixels per 256-thread block).
+struct TileMap {
+    int32_t width, height;
+    uint32_t tiles_x, ntiles, per_xcd;
+};
+
+struct TraceArgs {
+    vx_view view;
+    vx_trace_params params;
+    TileMap tiles;
+    const float* pos;
+    const float* nrm;
+    const uint8_t* mat;
+    const float* palette;
+    uint8_t* shadow;
+    float* ao;
+    float* spec;
+    uint32_t* fetches;
+};
+
+struct PrimaryArgs {
+    vx_view view;
+    TileMap tiles;
+    const uint8_t* mips[3];
+    int32_t dims[3];
+    float* pos;
+    float* nrm;
+    uint8_t* mat;
+};
+
+struct ShadeArgs {
+    vx_view view;
+    size_t npx;
+    const float* pos;
+    const float* nrm;
+    const uint8_t* mat;
+    const uint8_t* shadow;
+    const float* ao;
+    const float* ssao;
+    const float* palette;
+    float light[6];
+    float* rgb;
+    const float* dl_drgb;
+};
+
+__global__ void k_terrain_heights(float* h, int32_t nx, int32_t ny, int32_t nz, uint32_t seed, float fill);
+__global__ void k_terrain_fill(uint8_t* vol, const float* h, int32_t nx, int32_t ny, int32_t nz, uint32_t seed,
+                               int32_t use_terrain);
+__global__ void k_boxes(uint8_t* vol, int32_t nx, int32_t ny, int32_t nz, uint32_t seed, int32_t b0, int32_t emin,
+                        int32_t emax, int32_t grounded);
+__global__ void k_build_mip(const uint8_t* parent, int32_t px, int32_t py, int32_t pz, uint8_t* out);
+__global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v);
+__global__ void k_bytes_to_cells(uint64_t* cells, const uint8_t* bytes, OccView V, int32_t rx, int32_t ry, int32_t rz,
+                                 int32_t rw, int32_t rh, int32_t rd);
+__global__ void k_cells_to_bytes(uint8_t* bytes, const uint64_t* cells, OccView V);
+__global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, int32_t* aabb, int32_t thr);
+__global__ void k_level1(uint64_t* words, const uint64_t* cells, OccView V, int32_t w0x, int32_t w0y, int32_t w0z,
+                         int32_t nwx, int32_t nwy, int32_t nwz);
+__global__ void k_level_up(uint64_t* words, const uint64_t* lower, int32_t lwx, int32_t lwy, int32_t lwz, int32_t uwx,
+                           int32_t uwy, int32_t uwz, int32_t w0x, int32_t w0y, int32_t w0z, int32_t nwx, int32_t nwy,
+                           int32_t nwz);
+__global__ void k_trace_rays(OccView V, const float* rays, uint32_t n, int32_t march, vx_ray_result* out);
+__global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A);
+__global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
+__global__ void k_shade_fwd(ShadeArgs A);
+__global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);
+__global__ void k_shade_bwd_finish(const long long* acc, float* dpal, float* dlight);
+__global__ void k_zero_i64(long long* p, int n);
+
+}  // namespace vx

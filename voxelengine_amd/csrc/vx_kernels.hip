@@ -1,0 +1,824 @@
+// vx_kernels.hip -- gfx950 kernels of the voxel-traversal lighting path.
+//
+// Kernels:
+//   scene generation   k_terrain_heights, k_terrain_fill, k_boxes (build-defined inputs)
+//   model mips         k_build_mip            VoxAsset.cpp:16-53
+//   occupancy volume   k_fill_u64, k_bytes_to_cells, k_cells_to_bytes, k_footprint,
+//                      k_level1, k_level_up   ShadowVoxSystem.cpp:82-201
+//   primary hit        k_primary              GeometryVoxel.frag:49-183
+//   traversal          k_trace, k_trace_rays  Light.frag:14-217, LightAmbient.frag:111-175,
+//                                             LightReflection.frag:86-113
+//   shading            k_shade_fwd, k_shade_bwd, k_shade_bwd_finish
+//                                             LightAmbient.frag:178-214, PBR.frag:3-68
+#include "vx_internal.hpp"
+
+namespace vx {
+
+// ===========================================================================
+// Synthetic scenes. Terrain: y < H(x,z), H = 4-octave value-noise fBm with
+// the octave structure of Noise::GetOctave (Noise.cpp:93-107). Materials are
+// a hash of the voxel position (uniform 1..255), so box overlays commute.
+// ===========================================================================
+VXD uint32_t vox_material(uint32_t seed, int32_t x, int32_t y, int32_t z) {
+    uint32_t h = hash32(seed ^ 0x68bc21ebu);
+    h = hash32(h + (uint32_t)x);
+    h = hash32(h + (uint32_t)y);
+    h = hash32(h + (uint32_t)z);
+    return 1u + h % 255u;
+}
+VXD float lattice(uint32_t seed, int32_t ix, int32_t iz, int32_t octave) {
+    uint32_t h = hash32(seed + 0x9e3779b9u * (uint32_t)(octave + 1));
+    h = hash32(h + (uint32_t)ix);
+    h = hash32(h + (uint32_t)iz);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+VXD float value_noise(uint32_t seed, float fx, float fz, int32_t octave) {
+    float x0 = floorf(fx), z0 = floorf(fz);
+    int32_t ix = f2i(x0), iz = f2i(z0);
+    float tx = fx - x0, tz = fz - z0;
+    float sx = (tx * tx) * (3.0f - 2.0f * tx);
+    float sz = (tz * tz) * (3.0f - 2.0f * tz);
+    float v00 = lattice(seed, ix, iz, octave), v10 = lattice(seed, ix + 1, iz, octave);
+    float v01 = lattice(seed, ix, iz + 1, octave), v11 = lattice(seed, ix + 1, iz + 1, octave);
+    float a = v00 + (v10 - v00) * sx;
+    float b = v01 + (v11 - v01) * sx;
+    return a + (b - a) * sz;
+}
+
+__global__ void k_terrain_heights(float* h, int32_t nx, int32_t ny, int32_t nz, uint32_t seed, float fill) {
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * nz) return;
+    int32_t x = i % nx, z = i / nx;
+    float base_freq = 4.0f / (float)(nx > nz ? nx : nz);
+    float fx = (float)x + 0.5f, fz = (float)z + 0.5f;
+    float total = 0.0f, freq = base_freq, amp = 1.0f, maxv = 0.0f;
+    for (int32_t o = 0; o < 4; ++o) {
+        total = total + value_noise(seed, fx * freq, fz * freq, o) * amp;
+        maxv = maxv + amp;
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    h[i] = ((total / maxv) + (fill - 0.5f)) * (float)ny;
+}
+
+// One thread per 4 consecutive x voxels (one u32 store).
+__global__ void k_terrain_fill(uint8_t* vol, const float* h, int32_t nx, int32_t ny, int32_t nz, uint32_t seed,
+                               int32_t use_terrain) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nq = ((size_t)nx * ny * nz) / 4;
+    if (q >= nq) return;
+    size_t i0 = q * 4;
+    int32_t x = (int32_t)(i0 % nx);
+    size_t r = i0 / nx;
+    int32_t y = (int32_t)(r % ny), z = (int32_t)(r / ny);
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t m = 0;
+        if (use_terrain && (float)y < h[(size_t)z * nx + x + k]) m = vox_material(seed, x + k, y, z);
+        word |= m << (8 * k);
+    }
+    reinterpret_cast<uint32_t*>(vol)[q] = word;
+}
+
+__global__ void k_boxes(uint8_t* vol, int32_t nx, int32_t ny, int32_t nz, uint32_t seed, int32_t b0,
+                        int32_t emin, int32_t emax, int32_t grounded) {
+    int32_t b = b0 + blockIdx.y;
+    uint32_t base = hash32(seed ^ 0xb5297a4du);
+    uint32_t r[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r[k] = hash32(base + (uint32_t)b * 8u + (uint32_t)k);
+    const int32_t dims[3] = {nx, ny, nz};
+    int32_t e[3], o[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e[k] = emin + (int32_t)(r[k] % (uint32_t)(emax - emin + 1));
+        if (e[k] > dims[k]) e[k] = dims[k];
+        o[k] = (int32_t)(r[3 + k] % (uint32_t)(dims[k] - e[k] + 1));
+    }
+    if (grounded) o[1] = 0;
+    int32_t nvox = e[0] * e[1] * e[2];
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += gridDim.x * blockDim.x) {
+        int32_t x = o[0] + v % e[0];
+        int32_t y = o[1] + (v / e[0]) % e[1];
+        int32_t z = o[2] + v / (e[0] * e[1]);
+        vol[(size_t)x + (size_t)y * nx + (size_t)z * nx * ny] = (uint8_t)vox_material(seed, x, y, z);
+    }
+}
+
+// ===========================================================================
+// Model mips: VoxAsset::Upload (VoxAsset.cpp:26-47): child scan vi = 0..8
+// (vi = 8 re-reads child 0), first non-zero material wins.
+// ===========================================================================
+__global__ void k_build_mip(const uint8_t* parent, int32_t px, int32_t py, int32_t pz, uint8_t* out) {
+    int32_t mx = px > 1 ? px / 2 : 1, my = py > 1 ? py / 2 : 1, mz = pz > 1 ? pz / 2 : 1;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)mx * my * mz) return;
+    int32_t x = (int32_t)(i % mx);
+    int32_t y = (int32_t)((i / mx) % my);
+    int32_t z = (int32_t)(i / ((size_t)mx * my));
+    uint8_t vox = 0;
+    for (int vi = 0; vi < 9; ++vi) {
+        uint8_t u = parent[(size_t)(2 * x + ((vi >> 0) & 1)) + (size_t)(2 * y + ((vi >> 1) & 1)) * px +
+                           (size_t)(2 * z + ((vi >> 2) & 1)) * px * py];
+        if (u) { vox = u; break; }
+    }
+    out[i] = vox;
+}
+
+// ===========================================================================
+// Occupancy volume: layout conversion, rebuild, coarse levels.
+// ===========================================================================
+__global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Reference byte layout -> brick layout over a texel region (one thread per texel).
+__global__ void k_bytes_to_cells(uint64_t* cells, const uint8_t* bytes, OccView V, int32_t rx, int32_t ry,
+                                 int32_t rz, int32_t rw, int32_t rh, int32_t rd) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rw * rh * rd) return;
+    int32_t tx = rx + (int32_t)(i % rw);
+    int32_t ty = ry + (int32_t)((i / rw) % rh);
+    int32_t tz = rz + (int32_t)(i / ((size_t)rw * rh));
+    uint8_t b = bytes[(size_t)tx + (size_t)ty * V.sx + (size_t)tz * V.sx * V.sy];
+    size_t ci = brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1);
+    uint32_t sh = sub3(tx, ty, tz) * 8u;
+    // Eight texels share a cell word: update the byte with a read-modify-write
+    // on the 32-bit half that holds it.
+    uint32_t* w32 = reinterpret_cast<uint32_t*>(cells + ci) + (sh >> 5);
+    uint32_t s32 = sh & 31u;
+    atomicAnd(w32, ~(0xffu << s32));
+    atomicOr(w32, (uint32_t)b << s32);
+}
+
+// Brick layout -> reference byte layout (one thread per texel, whole volume).
+__global__ void k_cells_to_bytes(uint8_t* bytes, const uint64_t* cells, OccView V) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)V.sx * V.sy * V.sz) return;
+    int32_t tx = (int32_t)(i % V.sx);
+    int32_t ty = (int32_t)((i / V.sx) % V.sy);
+    int32_t tz = (int32_t)(i / ((size_t)V.sx * V.sy));
+    OccView W = V;
+    W.cells = cells;
+    bytes[i] = (uint8_t)texel_byte(W, tx, ty, tz);
+}
+
+// One footprint loop of ShadowVoxSystem::OnUpdate / OnVoxDestroyed
+// (ShadowVoxSystem.cpp:142-155, 164-177, 28-40) for a batch of operations that
+// commute (same value, or disjoint footprints). blockIdx.y = operation.
+__global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, int32_t* aabb, int32_t thr) {
+    const FootprintOp op = ops[blockIdx.y];
+    size_t nvox = (size_t)op.sx * op.sy * op.sz;
+    int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
+        uint8_t mat = op.model[i];
+        if (mat < thr) continue;
+        int32_t x = (int32_t)(i % op.sx);
+        int32_t y = (int32_t)((i / op.sx) % op.sy);
+        int32_t z = (int32_t)(i / ((size_t)op.sx * op.sy));
+        int32_t f[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float wp = ((op.o[k] + (float)x * op.dx[k]) + (float)y * op.dy[k]) + (float)z * op.dz[k];
+            f[k] = f2i(wp * 10.0f);
+            mn[k] = min(mn[k], f[k]);
+            mx[k] = max(mx[k], f[k]);
+        }
+        // SetVolumeAt (ShadowVoxSystem.cpp:82-94): out-of-range writes are dropped.
+        if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] >= 2 * V.sx || f[1] >= 2 * V.sy || f[2] >= 2 * V.sz) continue;
+        size_t ci = cell_index(V, f[0], f[1], f[2]);
+        uint32_t b = cell_bit(f[0], f[1], f[2]);
+        uint32_t* w32 = reinterpret_cast<uint32_t*>(cells + ci) + (b >> 5);
+        uint32_t m = 1u << (b & 31u);
+        if (op.value) atomicOr(w32, m);
+        else atomicAnd(w32, ~m);
+    }
+    // AABB of the footprint (ShadowVoxSystem.cpp:148-149): wave reduction, one
+    // atomic per wave into the instance's slot.
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int32_t a = mn[k], b = mx[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            a = min(a, __shfl_xor(a, off));
+            b = max(b, __shfl_xor(b, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            if (a != INT32_MAX) atomicMin(&aabb[op.inst * 6 + k], a);
+            if (b != INT32_MIN) atomicMax(&aabb[op.inst * 6 + 3 + k], b);
+        }
+    }
+}
+
+// Level 1: one bit per brick (64 B). One wave per level word: lane l handles
+// brick (4*wx + (l&3), 4*wy + ((l>>2)&3), 4*wz + (l>>4)); ballot packs the bits.
+__global__ void k_level1(uint64_t* words, const uint64_t* cells, OccView V, int32_t w0x, int32_t w0y, int32_t w0z,
+                         int32_t nwx, int32_t nwy, int32_t nwz) {
+    int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int32_t lane = threadIdx.x & 63;
+    if (wave >= nwx * nwy * nwz) return;
+    int32_t wx = w0x + wave % nwx, wy = w0y + (wave / nwx) % nwy, wz = w0z + wave / (nwx * nwy);
+    int32_t bx = 4 * wx + (lane & 3), by = 4 * wy + ((lane >> 2) & 3), bz = 4 * wz + (lane >> 4);
+    bool occ = false;
+    if (bx < V.bx && by < V.by && bz < V.bz) {
+        const uint64_t* b = cells + brick_index(V, bx, by, bz) * 8;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc |= b[c];
+        occ = acc != 0;
+    }
+    uint64_t m = __ballot(occ);
+    if (lane == 0) words[((size_t)wz * V.wy[0] + wy) * V.wx[0] + wx] = m;
+}
+
+// Level k >= 2 from level k-1: cell (cx,cy,cz) covers 2x2x2 words of level k-1.
+__global__ void k_level_up(uint64_t* words, const uint64_t* lower, int32_t lwx, int32_t lwy, int32_t lwz,
+                           int32_t uwx, int32_t uwy, int32_t uwz, int32_t w0x, int32_t w0y, int32_t w0z,
+                           int32_t nwx, int32_t nwy, int32_t nwz) {
+    int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int32_t lane = threadIdx.x & 63;
+    if (wave >= nwx * nwy * nwz) return;
+    int32_t wx = w0x + wave % nwx, wy = w0y + (wave / nwx) % nwy, wz = w0z + wave / (nwx * nwy);
+    int32_t cx = 4 * wx + (lane & 3), cy = 4 * wy + ((lane >> 2) & 3), cz = 4 * wz + (lane >> 4);
+    uint64_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        int32_t lx = 2 * cx + (j & 1), ly = 2 * cy + ((j >> 1) & 1), lz = 2 * cz + (j >> 2);
+        if (lx < lwx && ly < lwy && lz < lwz) acc |= lower[((size_t)lz * lwy + ly) * lwx + lx];
+    }
+    uint64_t m = __ballot(acc != 0);
+    if (lane == 0) words[((size_t)wz * uwy + wy) * uwx + wx] = m;
+}
+
+// ===========================================================================
+// Traversal
+// ===========================================================================
+
+// Coarse emptiness of brick (bx,by,bz): any level bit clear means every fine
+// voxel of the brick is empty (levels are OR-reductions), so the fetch of the
+// fine bit can be skipped without changing the result.
+VXD bool brick_is_empty(const OccView& V, int32_t bx, int32_t by, int32_t bz) {
+    for (int32_t l = V.nlevels - 1; l >= 0; --l) {
+        int32_t sh = 3 * l;
+        if (!level_bit(V, l, bx >> sh, by >> sh, bz >> sh)) return true;
+    }
+    return false;
+}
+
+struct RayOut {
+    int32_t hit, term, steps, fetches;
+    int32_t vox[3];
+    float t;
+    f3 hitpos, normal;
+};
+
+// raycastShadowVolume (Light.frag:29-81), mip 0 only (the ShadowVox image has
+// one level, evk.h:265). Passes 2..4 of the outer loop repeat pass 1 exactly,
+// so a pass that ends by n==256 or totalt>=maxt is traced once and reported
+// with nt*4 (Light.frag:38,74,79). Coarse levels only skip fine fetches.
+VXD void dda_exact(const OccView& V, f3 o, f3 d, float maxt, RayOut& R) {
+    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    float s[3], td[3], tm[3];
+    int32_t cv[3], up[3];
+    const int32_t dims[3] = {V.sx, V.sy, V.sz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s[k] = gsign(dd[k]);
+        td[k] = 1.0f / (dd[k] * s[k]);
+        cv[k] = f2i(floorf(oo[k]));
+        float nb = (float)cv[k] + (s[k] * 0.5f + 0.5f);
+        tm[k] = (nb - oo[k]) / dd[k];
+        up[k] = dims[k] * 2 + 1;
+    }
+    int32_t nt = 0, n = 0, term = 0;
+    float totalt = 0.0f;
+    int32_t bkx = -1, bky = -1, bkz = -1;
+    bool bempty = true;
+    size_t ckey = ~(size_t)0;
+    uint64_t cw = 0;
+    R.hit = 0;
+    do {
+        float s0 = gstep(tm[0], tm[2]) * gstep(tm[0], tm[1]);
+        float s1 = gstep(tm[1], tm[0]) * gstep(tm[1], tm[2]);
+        float s2 = gstep(tm[2], tm[1]) * gstep(tm[2], tm[0]);
+        if ((uint32_t)cv[0] > (uint32_t)up[0] || (uint32_t)cv[1] > (uint32_t)up[1] ||
+            (uint32_t)cv[2] > (uint32_t)up[2]) {
+            term = 2;
+            break;
+        }
+        int32_t bx = cv[0] >> 3, by = cv[1] >> 3, bz = cv[2] >> 3;
+        if (bx != bkx || by != bky || bz != bkz) {
+            bkx = bx; bky = by; bkz = bz;
+            bempty = brick_is_empty(V, bx, by, bz);
+        }
+        bool vox = false;
+        if (!bempty) {
+            size_t ci = cell_index(V, cv[0], cv[1], cv[2]);
+            if (ci != ckey) { ckey = ci; cw = V.cells[ci]; }
+            vox = (cw >> cell_bit(cv[0], cv[1], cv[2])) & 1ull;
+        }
+        float best_t = (tm[0] * s0 + tm[1] * s1) + tm[2] * s2;
+        if (vox) {
+            term = 1;
+            R.normal = mk3(-s[0] * s0, -s[1] * s1, -s[2] * s2);
+            R.hitpos = mk3(oo[0] + dd[0] * best_t, oo[1] + dd[1] * best_t, oo[2] + dd[2] * best_t);
+            R.vox[0] = cv[0]; R.vox[1] = cv[1]; R.vox[2] = cv[2];
+            R.t = best_t;
+            break;
+        }
+        cv[0] += f2i(s0 * s[0]);
+        cv[1] += f2i(s1 * s[1]);
+        cv[2] += f2i(s2 * s[2]);
+        tm[0] += td[0] * s0;
+        tm[1] += td[1] * s1;
+        tm[2] += td[2] * s2;
+        totalt = best_t;
+        nt++;
+    } while (++n < 256 && totalt < maxt);
+    if (term == 0) {
+        R.fetches = 4 * nt;
+        nt *= 4;
+    } else {
+        R.fetches = nt + (term == 1 ? 1 : 0);
+    }
+    R.hit = term == 1;
+    R.term = term;
+    R.steps = nt;
+}
+
+// raycastShadowVolumeSparse / ...SuperSparse (Light.frag:131-217).
+VXD float sparse_march(const OccView& V, f3 o, f3 d, float dist, float step_factor, RayOut& R) {
+    float sf = step_factor;
+    f3 sd = mk3(d.x * sf, d.y * sf, d.z * sf);
+    f3 pos = o;
+    float dd = sf;
+    int32_t it = 0;
+    R.hit = 0; R.term = 0;
+    while (dd < 16.0f) {
+        it++;
+        uint32_t vol = texel_byte(V, f2i(pos.x / 2.0f), f2i(pos.y / 2.0f), f2i(pos.z / 2.0f));
+        uint32_t bit = 0u;
+        bit += gmod(pos.x, 0.5f) > 0.25f ? 1u : 0u;
+        bit += gmod(pos.y, 0.5f) > 0.25f ? 2u : 0u;
+        bit += gmod(pos.z, 0.5f) > 0.25f ? 4u : 0u;
+        if (((1u << bit) & vol) != 0u) {
+            R.hit = 1; R.term = 1; R.steps = R.fetches = it; R.t = dd;
+            return dd;
+        }
+        pos.x += sd.x; pos.y += sd.y; pos.z += sd.z;
+        dd += sf;
+    }
+    sf *= 2.0f;
+    sd.x *= 2.0f; sd.y *= 2.0f; sd.z *= 2.0f;
+    float lim = gmin(dist, 164.0f);
+    while (dd < lim) {
+        it++;
+        // getVolumeAt(ivec3(pos), 1): byte of texel ivec3(pos)/2 is non-zero.
+        int32_t ix = f2i(pos.x), iy = f2i(pos.y), iz = f2i(pos.z);
+        if (texel_byte(V, ix / 2, iy / 2, iz / 2) != 0u) {
+            R.hit = 1; R.term = 1; R.steps = R.fetches = it; R.t = dd;
+            return dd;
+        }
+        pos.x += sd.x; pos.y += sd.y; pos.z += sd.z;
+        dd += sf;
+    }
+    R.steps = R.fetches = it;
+    R.t = dist;
+    return dist;
+}
+
+VXD void march_any(const OccView& V, int32_t march, f3 o, f3 d, float maxt, RayOut& R) {
+    if (march == VX_MARCH_DDA) dda_exact(V, o, d, maxt, R);
+    else sparse_march(V, o, d, maxt, march == VX_MARCH_SPARSE ? 0.5f : 2.5f, R);
+}
+
+// Distance a marcher reports for AO / spec-occ accumulation.
+VXD float march_dist(const OccView& V, int32_t march, f3 o, f3 d, float maxt, uint32_t& fetches, int32_t& hit) {
+    RayOut R;
+    march_any(V, march, o, d, maxt, R);
+    fetches += (uint32_t)R.fetches;
+    hit = R.hit;
+    if (march == VX_MARCH_DDA) return R.hit ? gmin(R.t, maxt) : maxt;
+    return R.t;
+}
+
+__global__ void k_trace_rays(OccView V, const float* rays, uint32_t n, int32_t march, vx_ray_result* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* ry = rays + (size_t)7 * i;
+    RayOut R;
+    R.vox[0] = R.vox[1] = R.vox[2] = 0;
+    R.t = 0.0f;
+    R.hitpos = mk3(0.0f, 0.0f, 0.0f);
+    R.normal = mk3(0.0f, 0.0f, 0.0f);
+    march_any(V, march, mk3(ry[0], ry[1], ry[2]), mk3(ry[3], ry[4], ry[5]), ry[6], R);
+    vx_ray_result o;
+    o.hit = R.hit; o.term = R.term; o.steps = R.steps; o.fetches = R.fetches;
+    if (march == VX_MARCH_DDA && R.hit) {
+        o.voxel[0] = R.vox[0]; o.voxel[1] = R.vox[1]; o.voxel[2] = R.vox[2];
+        o.t = R.t;
+        o.hitpos[0] = R.hitpos.x; o.hitpos[1] = R.hitpos.y; o.hitpos[2] = R.hitpos.z;
+        o.normal[0] = R.normal.x; o.normal[1] = R.normal.y; o.normal[2] = R.normal.z;
+    } else {
+        o.voxel[0] = o.voxel[1] = o.voxel[2] = 0;
+        o.t = (march == VX_MARCH_DDA) ? 0.0f : R.t;
+        o.hitpos[0] = o.hitpos[1] = o.hitpos[2] = 0.0f;
+        o.normal[0] = o.normal[1] = o.normal[2] = 0.0f;
+    }
+    out[i] = o;
+}
+
+// Pixel of a 16x16 block tile; blocks are remapped so each XCD (blockIdx % 8)
+// walks a contiguous band of tiles.
+VXD bool tile_pixel(const TileMap& tm, int32_t& px, int32_t& py) {
+    uint32_t b = blockIdx.x;
+    uint32_t logical = (b % 8u) * tm.per_xcd + b / 8u;
+    if (logical >= tm.ntiles) return false;
+    int32_t tx = (int32_t)(logical % tm.tiles_x), ty = (int32_t)(logical / tm.tiles_x);
+    int32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    px = tx * 16 + (wave & 1) * 8 + (lane & 7);
+    py = ty * 16 + (wave >> 1) * 8 + (lane >> 3);
+    return px < tm.width && py < tm.height;
+}
+
+// Ambient-pass pixel (LightAmbient.frag:134-175) generalised to n_ao cosine AO
+// rays with a length cap, plus a GGX-sampled specular-occlusion ray
+// (LightReflection.frag:86-113 ray origin).
+__global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A) {
+    int32_t px, py;
+    if (!tile_pixel(A.tiles, px, py)) return;
+    const vx_trace_params& P = A.params;
+    size_t i = (size_t)py * A.view.width + px;
+    uint8_t m = A.mat[i];
+    if (m == 0) {
+        A.shadow[i] = 0; A.ao[i] = 0.0f; A.spec[i] = 0.0f;
+        if (A.fetches) A.fetches[i] = 0;
+        return;
+    }
+    uint32_t nf = 0;
+    f3 pp = load3(A.pos + 3 * i), n = load3(A.nrm + 3 * i);
+    uint32_t pk = pixkey(P.seed, P.frame, (uint32_t)i);
+    float r0[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r0[k] = rand01(pk, (uint32_t)k);
+
+    float wx = pp.x * 0.1f, wy = pp.y * 0.1f, wz = pp.z * 0.1f;
+    const float* Vm = A.view.view;
+    float vz = ((Vm[2] * wx + Vm[6] * wy) + Vm[10] * wz) + Vm[14];
+    float depth = ((-vz) - 0.1f) / (4096.0f - 0.1f);
+    float st = gclamp((depth - 0.0f) / (0.2f - 0.0f), 0.0f, 1.0f);
+    float bias = (st * st) * (3.0f - 2.0f * st) * 50.0f + 1.5f;
+
+    f3 wd = mk3(P.light_dir[0], P.light_dir[1], P.light_dir[2]);
+    f3 wcp = pp;
+    if (P.jitter_shadow) {
+        f3 rv = cosine_hemisphere(r0[0], r0[1]);
+        rv = mk3(rv.x * 0.1f, rv.y * 0.1f, rv.z * 0.1f);
+        rv.z = rv.z * gsign(r0[2] - 0.5f);
+        f3 mx = mk3(wd.x * (1.0f - 0.5f) + rv.x * 0.5f, wd.y * (1.0f - 0.5f) + rv.y * 0.5f,
+                    wd.z * (1.0f - 0.5f) + rv.z * 0.5f);
+        wd = normalize3(mx);
+        float w = r0[3] * 1.0f;
+        wcp = mk3(wcp.x + wd.x * w, wcp.y + wd.y * w, wcp.z + wd.z * w);
+        wcp = mk3(wcp.x + rv.x * 2.5f, wcp.y + rv.y * 2.5f, wcp.z + rv.z * 2.5f);
+    }
+    f3 org = mk3(wcp.x + n.x * bias, wcp.y + n.y * bias, wcp.z + n.z * bias);
+
+    uint8_t lit = 1;
+    if (P.n_shadow > 0) {
+        int32_t h;
+        float t = march_dist(V, P.shadow_march, org, wd, P.shadow_maxt, nf, h);
+        if (P.shadow_march == VX_MARCH_DDA) lit = h ? 0 : 1;
+        else lit = (t != P.shadow_maxt) ? 0 : 1;
+    }
+    A.shadow[i] = lit;
+
+    float aov = 1.0f;
+    if (P.n_ao > 0) {
+        f3 T, B;
+        tangent_frame(n, T, B);
+        float aosum = 0.0f;
+        for (int32_t a = 0; a < P.n_ao; ++a) {
+            float u1 = rand01(pk, (uint32_t)(4 * (1 + a) + 0));
+            float u2 = rand01(pk, (uint32_t)(4 * (1 + a) + 1));
+            f3 rv = cosine_hemisphere(u1, u2);
+            f3 dir = mk3((T.x * rv.x + B.x * rv.y) + n.x * rv.z, (T.y * rv.x + B.y * rv.y) + n.y * rv.z,
+                         (T.z * rv.x + B.z * rv.y) + n.z * rv.z);
+            int32_t h;
+            float t = march_dist(V, P.ao_march, org, dir, P.ao_maxt, nf, h);
+            float q = gmin(t, P.ao_maxt) / P.ao_maxt;
+            aosum = aosum + q * q;
+        }
+        aov = aosum / (float)P.n_ao;
+    }
+    A.ao[i] = aov;
+
+    float spv = 1.0f;
+    if (P.n_spec > 0) {
+        uint32_t base = (uint32_t)(4 * (1 + P.n_ao));
+        float u1 = rand01(pk, base + 0), u2 = rand01(pk, base + 1), u3 = rand01(pk, base + 3);
+        float rough = A.palette[6 * m + 3];
+        f3 cam = mk3(A.view.cam_pos[0] * 10.0f, A.view.cam_pos[1] * 10.0f, A.view.cam_pos[2] * 10.0f);
+        f3 Vd = normalize3(mk3(cam.x - pp.x, cam.y - pp.y, cam.z - pp.z));
+        float alpha = rough * rough;
+        float a2 = alpha * alpha;
+        float ct2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
+        float ct = sqrtf(ct2);
+        float stt = sqrtf(gmax(0.0f, 1.0f - ct2));
+        float sp, cp;
+        sincos_vx(6.2831855f * u2, &sp, &cp);
+        f3 T, B;
+        tangent_frame(n, T, B);
+        f3 hl = mk3(stt * cp, stt * sp, ct);
+        f3 h = mk3((T.x * hl.x + B.x * hl.y) + n.x * hl.z, (T.y * hl.x + B.y * hl.y) + n.y * hl.z,
+                   (T.z * hl.x + B.z * hl.y) + n.z * hl.z);
+        f3 I = mk3(-Vd.x, -Vd.y, -Vd.z);
+        float dhi = dot3(h, I);
+        f3 wdir = mk3(I.x - (2.0f * dhi) * h.x, I.y - (2.0f * dhi) * h.y, I.z - (2.0f * dhi) * h.z);
+        f3 so = mk3((pp.x + n.x * u3) + n.x, (pp.y + n.y * u3) + n.y, (pp.z + n.z * u3) + n.z);
+        int32_t hh;
+        float t = march_dist(V, P.spec_march, so, wdir, P.spec_maxt, nf, hh);
+        spv = gmin(t, P.spec_maxt) / P.spec_maxt;
+    }
+    A.spec[i] = spv;
+    if (A.fetches) A.fetches[i] = nf;
+}
+
+// ===========================================================================
+// Primary hit: GeometryVoxel.frag (LOD 1, MIP_COUNT 2, TRANSPARENCY 1) for the
+// world model with an identity WorldMatrix.
+// ===========================================================================
+VXD uint32_t mip_fetch(const PrimaryArgs& A, int32_t mip, int32_t x, int32_t y, int32_t z) {
+    int32_t mx = A.dims[0] >> mip, my = A.dims[1] >> mip, mz = A.dims[2] >> mip;
+    if ((uint32_t)x >= (uint32_t)mx || (uint32_t)y >= (uint32_t)my || (uint32_t)z >= (uint32_t)mz) return 0u;
+    return A.mips[mip][(size_t)x + (size_t)y * mx + (size_t)z * mx * my];
+}
+
+// intersectVolume, GeometryVoxel.frag:63-125.
+VXD int32_t intersect_volume(const PrimaryArgs& A, f3 o3, f3 d3, f3 cam, float uvx, float uvy, f3& hit,
+                             f3& nrm, uint32_t& material) {
+    float o[3] = {o3.x, o3.y, o3.z}, d[3] = {d3.x, d3.y, d3.z}, c[3] = {cam.x, cam.y, cam.z};
+    float s[3], td[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s[k] = gsign(d[k]);
+        td[k] = 1.0f / (d[k] * s[k]);
+    }
+    int32_t mip = 2;
+    int32_t i = 0;
+    do {
+        float mip_size = (float)(1 << mip);
+        int32_t cv[3], up[3];
+        float tm[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            o[k] = o[k] / mip_size;
+            cv[k] = f2i(floorf(o[k]));
+            float nb = (float)cv[k] + (s[k] * 0.5f + 0.5f);
+            tm[k] = (nb - o[k]) / d[k];
+            up[k] = f2i((float)A.dims[k] / mip_size) + 1;
+        }
+        int32_t n = 0;
+        do {
+            float s0 = gstep(tm[0], tm[2]) * gstep(tm[0], tm[1]);
+            float s1 = gstep(tm[1], tm[0]) * gstep(tm[1], tm[2]);
+            float s2 = gstep(tm[2], tm[1]) * gstep(tm[2], tm[0]);
+            cv[0] += f2i(s0 * s[0]);
+            cv[1] += f2i(s1 * s[1]);
+            cv[2] += f2i(s2 * s[2]);
+            if (cv[0] < -1 || cv[0] > up[0] || cv[1] < -1 || cv[1] > up[1] || cv[2] < -1 || cv[2] > up[2])
+                return 0;
+            uint32_t vox = mip_fetch(A, mip, cv[0], cv[1], cv[2]);
+            if (vox != 0u) {
+                float best_t = (tm[0] * s0 + tm[1] * s1) + tm[2] * s2;
+                bool accept = (mip == 0);
+                if (!accept) {
+                    f3 df = mk3(c[0] - (o[0] + d[0] * best_t) * mip_size, c[1] - (o[1] + d[1] * best_t) * mip_size,
+                                c[2] - (o[2] + d[2] * best_t) * mip_size);
+                    accept = (float)mip < 0.001f * sqrtf(dot3(df, df));
+                }
+                if (accept) {
+                    float cx = roundf((uvx * A.view.res[0]) * 0.5f);
+                    float cy = roundf((uvy * A.view.res[1]) * 0.5f);
+                    bool glass = vox < 16u && gmod(cy + cx, 2.0f) == (float)(A.view.frame % 2);
+                    if (!glass) {
+                        nrm = mk3(-s[0] * s0, -s[1] * s1, -s[2] * s2);
+                        hit = mk3((o[0] + d[0] * best_t) * mip_size, (o[1] + d[1] * best_t) * mip_size,
+                                  (o[2] + d[2] * best_t) * mip_size);
+                        material = vox;
+                        return 1;
+                    }
+                } else {
+                    mip--;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) o[k] = o[k] + d[k] * best_t / mip_size;
+                    break;
+                }
+            }
+            tm[0] += td[0] * s0;
+            tm[1] += td[1] * s1;
+            tm[2] += td[2] * s2;
+        } while (++n < 512);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[k] = o[k] * mip_size;
+    } while (++i < 4);
+    return 0;
+}
+
+__global__ void __launch_bounds__(256) k_primary(PrimaryArgs A) {
+    int32_t px, py;
+    if (!tile_pixel(A.tiles, px, py)) return;
+    const vx_view& vw = A.view;
+    size_t i = (size_t)py * vw.width + px;
+    // Pinhole ray from the inverse projection / view (LightAmbient.vert:32-36).
+    float u = ((float)px + 0.5f + vw.jitter[0]) / vw.res[0];
+    float v = ((float)py + 0.5f + vw.jitter[1]) / vw.res[1];
+    float ndc_x = u * 2.0f - 1.0f, ndc_y = 1.0f - v * 2.0f;
+    float c4[4] = {ndc_x, ndc_y, 1.0f, 1.0f}, f4[4], w4[4];
+    mat_vec(vw.inv_proj, c4, f4);
+    float fv[4] = {f4[0] / f4[3], f4[1] / f4[3], f4[2] / f4[3], 0.0f};
+    mat_vec(vw.inv_view, fv, w4);
+    f3 dir = normalize3(mk3(w4[0], w4[1], w4[2]));
+    f3 cam = mk3(vw.cam_pos[0] * 10.0f, vw.cam_pos[1] * 10.0f, vw.cam_pos[2] * 10.0f);
+    // clipToAABB, GeometryVoxel.frag:49-60.
+    float size[3] = {(float)A.dims[0], (float)A.dims[1], (float)A.dims[2]};
+    float co[3] = {cam.x, cam.y, cam.z}, cd[3] = {dir.x, dir.y, dir.z};
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (gclamp(co[k], 0.0f, size[k]) != co[k]) inside = false;
+    f3 start = cam;
+    if (!inside) {
+        float t[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float inv = 1.0f / cd[k];
+            float sgn = gstep(cd[k], 0.0f);
+            t[k] = (sgn * size[k] - co[k]) * inv;
+        }
+        float tmin = gmax(gmax(t[0], t[1]), t[2]);
+        start = mk3(co[0] + cd[0] * (tmin - 0.001f), co[1] + cd[1] * (tmin - 0.001f), co[2] + cd[2] * (tmin - 0.001f));
+    }
+    f3 hit = mk3(0.0f, 0.0f, 0.0f), nrm = mk3(0.0f, 0.0f, 0.0f);
+    uint32_t mat = 0;
+    int32_t h = intersect_volume(A, start, dir, cam, ndc_x, ndc_y, hit, nrm, mat);
+    if (!h) {
+        hit = mk3(0.0f, 0.0f, 0.0f);
+        nrm = mk3(0.0f, 0.0f, 0.0f);
+        mat = 0;
+    }
+    A.pos[3 * i + 0] = hit.x; A.pos[3 * i + 1] = hit.y; A.pos[3 * i + 2] = hit.z;
+    A.nrm[3 * i + 0] = nrm.x; A.nrm[3 * i + 1] = nrm.y; A.nrm[3 * i + 2] = nrm.z;
+    A.mat[i] = (uint8_t)mat;
+}
+
+// ===========================================================================
+// Shading (LightAmbient.frag:178-214, PBRDirectLight PBR.frag:48-68,
+// fresnelSchlickRoughness PBR.frag:8-10) and its adjoint.
+// ===========================================================================
+struct ShadePix {
+    f3 N, V;
+    float sh, A, s5, ndl, nl;
+};
+
+VXD void sky_irradiance(float irr[3]) {
+    // getSkyColor(vec3(1,0,0)), LightAmbient.frag:128-131.
+    float ey = (gmax(0.0f, 0.0f) * 0.8f + 0.2f) * 0.8f;
+    irr[0] = ((1.0f - ey) * (1.0f - ey)) * 1.1f;
+    irr[1] = (1.0f - ey) * 1.1f;
+    irr[2] = (0.6f + (1.0f - ey) * 0.4f) * 1.1f;
+}
+
+VXD void shade_setup(const ShadeArgs& A, size_t i, f3 L, ShadePix& S) {
+    f3 pp = load3(A.pos + 3 * i);
+    S.N = load3(A.nrm + 3 * i);
+    f3 cam = mk3(A.view.cam_pos[0] * 10.0f, A.view.cam_pos[1] * 10.0f, A.view.cam_pos[2] * 10.0f);
+    S.V = normalize3(mk3(cam.x - pp.x, cam.y - pp.y, cam.z - pp.z));
+    S.sh = (float)A.shadow[i];
+    S.A = (A.ao[i] * 0.05f) * (A.ssao ? A.ssao[i] : 1.0f);
+    float ndv = gmax(dot3(S.N, S.V), 0.0f);
+    float x = gmax(1.0f - ndv, 0.0f);
+    S.s5 = (((x * x) * x) * x) * x;
+    S.nl = dot3(S.N, L);
+    S.ndl = gmax(S.nl, 0.0f);
+}
+
+__global__ void k_shade_fwd(ShadeArgs A) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.npx) return;
+    uint32_t m = A.mat[i];
+    if (m == 0) {
+        A.rgb[3 * i] = 0.0f; A.rgb[3 * i + 1] = 0.0f; A.rgb[3 * i + 2] = 0.0f;
+        return;
+    }
+    float irr[3];
+    sky_irradiance(irr);
+    f3 L = normalize3(mk3(A.light[0], A.light[1], A.light[2]));
+    ShadePix S;
+    shade_setup(A, i, L, S);
+    const float* e = A.palette + 6 * m;
+    float rough = e[3], metal = e[4], emit = e[5];
+    float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
+    float F = f0d + (1.0f - f0d) * S.s5;
+    float kD = (1.0f - F) * (1.0f - metal);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float rad = A.light[3 + c] * S.sh;
+        float lo = (((kD * e[c]) / 3.14159265359f) * rad) * S.ndl;
+        float f0a = 0.04f * (1.0f - metal) + e[c] * metal;
+        float mx = gmax(1.0f - rough, f0a);
+        float fr = f0a + (mx - f0a) * S.s5;
+        float kda = 1.0f - fr;
+        float amb = (irr[c] * e[c]) * (emit * 10.0f + (kda * S.A));
+        A.rgb[3 * i + c] = amb + lo;
+    }
+}
+
+// Fixed-point accumulation: 2^30 units per 1.0 in int64, so the sum is
+// independent of the order of the atomics (deterministic gradients).
+VXD long long to_fixed(float v) { return __float2ll_rn(v * 1073741824.0f); }
+
+// Adjoint of k_shade_fwd with visibility fixed. Per-pixel partials are summed
+// into an LDS histogram of 256x6 (+6 light) int64 fixed-point cells, then
+// flushed with int64 atomics into the global accumulator acc[256*6+6].
+__global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc) {
+    __shared__ long long hist[256 * 6 + 6];
+    for (int j = threadIdx.x; j < 256 * 6 + 6; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    float irr[3];
+    sky_irradiance(irr);
+    f3 ld = mk3(A.light[0], A.light[1], A.light[2]);
+    float ll = sqrtf(dot3(ld, ld));
+    f3 L = mk3(ld.x / ll, ld.y / ll, ld.z / ll);
+    float gl[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.npx; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t m = A.mat[i];
+        if (m == 0) continue;
+        ShadePix S;
+        shade_setup(A, i, L, S);
+        const float* e = A.palette + 6 * m;
+        float rough = e[3], metal = e[4];
+        float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
+        float F = f0d + (1.0f - f0d) * S.s5;
+        float kD = (1.0f - F) * (1.0f - metal);
+        float dF_dme = 0.96f * (1.0f - S.s5);
+        float dkD_dme = -dF_dme * (1.0f - metal) - (1.0f - F);
+        float gp[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float gc = A.dl_drgb[3 * i + c];
+            float a = e[c];
+            float rad = A.light[3 + c] * S.sh;
+            float f0a = 0.04f * (1.0f - metal) + a * metal;
+            bool pick = (1.0f - rough) < f0a;
+            float mx = pick ? f0a : (1.0f - rough);
+            float fr = f0a + (mx - f0a) * S.s5;
+            float E = e[5] * 10.0f + (1.0f - fr) * S.A;
+            float dlo_da = ((kD / 3.14159265359f) * rad) * S.ndl;
+            float dmx_da = pick ? metal : 0.0f;
+            float dfr_da = metal + (dmx_da - metal) * S.s5;
+            gp[c] = gc * (dlo_da + (irr[c] * E - ((irr[c] * a) * S.A) * dfr_da));
+            float dfr_dro = pick ? 0.0f : -S.s5;
+            gp[3] += gc * (-((irr[c] * a) * S.A) * dfr_dro);
+            float dlo_dme = ((dkD_dme * a / 3.14159265359f) * rad) * S.ndl;
+            float df0a_dme = a - 0.04f;
+            float dmx_dme = pick ? df0a_dme : 0.0f;
+            float dfr_dme = df0a_dme + (dmx_dme - df0a_dme) * S.s5;
+            gp[4] += gc * (dlo_dme - ((irr[c] * a) * S.A) * dfr_dme);
+            gp[5] += gc * ((irr[c] * a) * 10.0f);
+            gl[3 + c] += gc * ((((kD * a) / 3.14159265359f) * S.sh) * S.ndl);
+            if (!(S.nl < 0.0f)) {
+                float k = gc * (((kD * a) / 3.14159265359f) * rad) / ll;
+                gl[0] += k * (S.N.x - L.x * S.nl);
+                gl[1] += k * (S.N.y - L.y * S.nl);
+                gl[2] += k * (S.N.z - L.z * S.nl);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) atomicAdd((unsigned long long*)&hist[6 * m + k], (unsigned long long)to_fixed(gp[k]));
+    }
+    // Light partials: per-lane fixed point, wave sum in int64, one LDS atomic per wave.
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        long long v = to_fixed(gl[k]);
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&hist[256 * 6 + k], (unsigned long long)v);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 256 * 6 + 6; j += blockDim.x)
+        if (hist[j] != 0) atomicAdd((unsigned long long*)&acc[j], (unsigned long long)hist[j]);
+}
+
+__global__ void k_shade_bwd_finish(const long long* acc, float* dpal, float* dlight) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < 256 * 6) dpal[j] = (float)((double)acc[j] * (1.0 / 1073741824.0));
+    else if (j < 256 * 6 + 6) dlight[j - 256 * 6] = (float)((double)acc[j] * (1.0 / 1073741824.0));
+}
+
+__global__ void k_zero_i64(long long* p, int n) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) p[j] = 0;
+}
+
+}  // namespace vx
