@@ -52,6 +52,59 @@ def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
             "sample": f"every {stride}th hit pixel ({sample.size} px, {rays} rays, {dt:.1f} s)"}
 
 
+def bench_batched(ctx, pal_d, reps=5):
+    """64 scenes of uneven size (each axis a multiple of 8 in [64,512]), one view
+    each (256x256..1920x1080), 1 shadow + 8 AO rays: one graph-captured
+    persistent launch against 64 plain per-view launches."""
+    import torch
+    from voxelengine_amd import api, configs, views
+    params = configs.trace_params(n_shadow=1, n_ao=8, ao_maxt=32.0, n_spec=0, march="dda")
+    items = []
+    for sc in configs.batched_scenes():
+        m = api.Model(ctx, *sc["dims"])
+        m.generate(seed=sc["seed"], terrain_fill=sc["fill"], nboxes=sc["nboxes"], box_min=sc["box"][0],
+                   box_max=sc["box"][1])
+        m.build_mips()
+        d = m.dims
+        occ = api.Volume(ctx, d[0] // 2, d[1] // 2, d[2] // 2, nlevels=2)
+        occ.clear()
+        occ.rebuild([api.make_instance(m.mip_ptr(0), d, api.IDENTITY, api.IDENTITY)])
+        occ.build_levels()
+        vd = views.scene_view(d, sc["width"], sc["height"], variant=sc["variant"])
+        g = api.GBuffer(sc["width"], sc["height"])
+        api.primary(ctx, m, vd, g)
+        items.append((m, occ, vd, g, api.Visibility(sc["width"], sc["height"])))
+    torch.cuda.synchronize()
+    hit = sum(int((it[3].material > 0).sum().item()) for it in items)
+    rays = hit * configs.rays_per_pixel(params)
+    bvis = [api.Visibility(it[2]["width"], it[2]["height"]) for it in items]
+    batch = api.Batch(ctx, [(it[1], it[2], it[3], bv) for it, bv in zip(items, bvis)], pal_d, params)
+
+    def plain():
+        for (m, occ, vd, g, vis) in items:
+            api.trace(ctx, occ, vd, g, pal_d, params, vis)
+
+    res = {}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, fn in (("plain_64_launches", plain), ("batched_graph", batch.launch)):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": round(ms, 3), "grays_per_s": round(rays / (ms * 1e-3) / 1e9, 4)}
+    same = all(torch.equal(a[4].ao.view(torch.int32), b.ao.view(torch.int32)) and torch.equal(a[4].shadow, b.shadow)
+               for a, b in zip(items, bvis))
+    res["rays"] = rays
+    res["batched_vs_plain"] = round(res["plain_64_launches"]["ms"] / res["batched_graph"]["ms"], 3)
+    res["identical_outputs"] = bool(same)
+    batch.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +241,9 @@ def main():
         bwd_ms = ev0.elapsed_time(ev1) / 20
         out["backward"] = {"ms": round(bwd_ms, 4), "grays_per_s": round(rays_per_frame / (bwd_ms * 1e-3) / 1e9, 2),
                            "px_per_s": round(hit_px / (bwd_ms * 1e-3), 1)}
+
+    if not args.no_extras:
+        out["batched"] = bench_batched(ctx, pal_d)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -1,0 +1,230 @@
+// passes.hpp -- C++17 host wrappers over the C ABI that keep the reference's
+// system and render-pass shapes, so engine code written against VoxelEngine's
+// ShadowVoxSystem (Sources/World/Systems/ShadowVoxSystem.h:9-37) and
+// Light*Pipeline::Use (Sources/Graphics/Pipelines/LightAmbientPipeline.h:35,
+// LightReflectionPipeline.h:34) drops onto the gfx950 path.
+//
+// Header-only; link with libvx_amd.so. Errors throw vx::Error, as the
+// reference's CHECK throws (Sources/Core/Core.h:83-86).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vx_api.h"
+
+namespace vx {
+
+class Error : public std::runtime_error {
+public:
+    Error(vx_status code, const char* msg) : std::runtime_error(msg ? msg : "vx error"), code(code) {}
+    vx_status code;
+};
+
+inline void check(vx_status s) {
+    if (s != VX_OK) throw Error(s, vx_last_error());
+}
+
+// Graphics module start-up for this path: one HIP device.
+class Context {
+public:
+    explicit Context(int device = 0) { check(vx_ctx_create(device, &h_)); }
+    ~Context() { vx_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    vx_ctx* get() const { return h_; }
+
+private:
+    vx_ctx* h_ = nullptr;
+};
+
+// VoxAsset (Sources/Asset/VoxAsset.h:8-62): material volume with 3 mips.
+class VoxAsset {
+public:
+    VoxAsset(Context& ctx, int sx, int sy, int sz) { check(vx_model_create(ctx.get(), sx, sy, sz, &h_)); }
+    ~VoxAsset() { vx_model_destroy(h_); }
+    VoxAsset(const VoxAsset&) = delete;
+    VoxAsset& operator=(const VoxAsset&) = delete;
+    // VoxAsset::Upload (VoxAsset.cpp:3-64): level 0 copy, then the mip build.
+    void Upload(const std::vector<uint8_t>& data, void* stream = nullptr) {
+        check(vx_model_upload(h_, data.data(), data.size(), stream));
+        check(vx_model_build_mips(h_, stream));
+    }
+    void Generate(const vx_scene_desc& d, void* stream = nullptr) {
+        check(vx_model_generate(h_, &d, stream));
+        check(vx_model_build_mips(h_, stream));
+    }
+    std::array<int32_t, 3> Extent() const {
+        std::array<int32_t, 3> d{};
+        check(vx_model_dims(h_, d.data()));
+        return d;
+    }
+    const uint8_t* Data() const {
+        uint8_t* p = nullptr;
+        check(vx_model_mip_ptr(h_, 0, &p));
+        return p;
+    }
+    vx_model* GetImage() const { return h_; }
+
+private:
+    vx_model* h_ = nullptr;
+};
+
+using Entity = uint32_t;
+
+// Transform + VoxRenderer of one entity (Sources/World/Components.h:30-35,60-67).
+struct VoxEntity {
+    const VoxAsset* Vox = nullptr;
+    float Pivot[3] = {0.0f, 0.0f, 0.0f};
+    float PreviousWorldMatrix[16];
+    float WorldMatrix[16];
+    bool Changed = true;
+};
+
+// ShadowVoxSystem (ShadowVoxSystem.h:9-37): owns the occupancy volume and
+// rebuilds it from the entities changed since the last update.
+class ShadowVoxSystem {
+public:
+    // Reference volume: 524 x 188 x 524 texels (ShadowVoxSystem.cpp:56-58).
+    explicit ShadowVoxSystem(Context& ctx, int sx = 524, int sy = 188, int sz = 524, int levels = 2)
+        : ctx_(ctx) {
+        check(vx_volume_create(ctx.get(), sx, sy, sz, levels, &vol_));
+        check(vx_volume_clear(vol_, nullptr));
+    }
+    ~ShadowVoxSystem() { vx_volume_destroy(vol_); }
+    ShadowVoxSystem(const ShadowVoxSystem&) = delete;
+    ShadowVoxSystem& operator=(const ShadowVoxSystem&) = delete;
+
+    void OnCreate() {}
+    void OnDestroy() {}
+
+    Entity CreateEntity(const VoxAsset* vox, const float world[16], const float pivot[3] = nullptr) {
+        VoxEntity e;
+        e.Vox = vox;
+        if (pivot) std::memcpy(e.Pivot, pivot, sizeof(e.Pivot));
+        static const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        std::memcpy(e.PreviousWorldMatrix, I, sizeof(I));
+        std::memcpy(e.WorldMatrix, world, sizeof(e.WorldMatrix));
+        e.Changed = true;
+        Entity id = next_++;
+        entities_[id] = e;
+        return id;
+    }
+
+    // TransformSystem: PreviousWorldMatrix <- WorldMatrix, new WorldMatrix, Changed tag
+    // (TransformSystem.cpp:35-71).
+    void SetWorldMatrix(Entity id, const float world[16]) {
+        VoxEntity& e = entities_.at(id);
+        if (!e.Changed) std::memcpy(e.PreviousWorldMatrix, e.WorldMatrix, sizeof(e.WorldMatrix));
+        std::memcpy(e.WorldMatrix, world, sizeof(e.WorldMatrix));
+        e.Changed = true;
+    }
+
+    // OnVoxDestroyed (ShadowVoxSystem.cpp:7-53): erase with the WorldMatrix and no
+    // pivot; the erase runs before the next update's entities.
+    void OnVoxDestroyed(Entity id) {
+        auto it = entities_.find(id);
+        if (it == entities_.end()) return;
+        const VoxEntity& e = it->second;
+        if (e.Vox) pending_.push_back(instance(e, VX_INST_DESTROY));
+        entities_.erase(it);
+    }
+
+    // OnUpdate (ShadowVoxSystem.cpp:116-201): every Changed entity in order, then
+    // the coarse levels over the dirty regions.
+    void OnUpdate(float /*dt*/, void* stream = nullptr) {
+        std::vector<vx_instance> inst = std::move(pending_);
+        pending_.clear();
+        for (auto& kv : entities_) {
+            VoxEntity& e = kv.second;
+            if (!e.Changed) continue;
+            if (e.Vox) inst.push_back(instance(e, VX_INST_UPDATE));
+            e.Changed = false;
+            std::memcpy(e.PreviousWorldMatrix, e.WorldMatrix, sizeof(e.WorldMatrix));
+        }
+        if (inst.empty()) return;
+        regions_.assign(inst.size(), vx_region{});
+        uint32_t nd = 0;
+        check(vx_volume_rebuild(vol_, inst.data(), (uint32_t)inst.size(), 16, regions_.data(), &nd, stream));
+        regions_.resize(nd);
+        if (nd) check(vx_volume_build_levels(vol_, regions_.data(), nd, stream));
+    }
+
+    const std::vector<vx_region>& LastDirtyRegions() const { return regions_; }
+    vx_volume* GetVolumeImage() const { return vol_; }
+
+private:
+    static vx_instance instance(const VoxEntity& e, int32_t flags) {
+        vx_instance in{};
+        std::array<int32_t, 3> d = e.Vox->Extent();
+        in.model = e.Vox->Data();
+        in.sx = d[0]; in.sy = d[1]; in.sz = d[2];
+        std::memcpy(in.prev, e.PreviousWorldMatrix, sizeof(in.prev));
+        std::memcpy(in.cur, e.WorldMatrix, sizeof(in.cur));
+        std::memcpy(in.pivot, e.Pivot, sizeof(in.pivot));
+        in.flags = flags;
+        return in;
+    }
+
+    Context& ctx_;
+    vx_volume* vol_ = nullptr;
+    std::map<Entity, VoxEntity> entities_;
+    std::vector<vx_instance> pending_;
+    std::vector<vx_region> regions_;
+    Entity next_ = 1;
+};
+
+// GeometryVoxelPipeline (GeometryVoxelPipeline.h:15-76) for the world model:
+// fills the fp32 G-buffer.
+class GeometryVoxelPipeline {
+public:
+    static GeometryVoxelPipeline& Get() {
+        static GeometryVoxelPipeline p;
+        return p;
+    }
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const VoxAsset& world, const vx_gbuf& geometryFB) {
+        check(vx_primary(ctx.get(), world.GetImage(), &viewBuffer, &geometryFB, cmd));
+    }
+};
+
+// LightAmbientPipeline (LightAmbientPipeline.h:12-58): sun shadow and voxel AO
+// rays of the ambient pass; the shading itself is vx_shade_forward.
+class LightAmbientPipeline {
+public:
+    static LightAmbientPipeline& Get() {
+        static LightAmbientPipeline p;
+        return p;
+    }
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB, vx_volume* shadowVox,
+             const float* palette, const vx_trace_params& params, const vx_trace_out& lightOut) {
+        vx_trace_params p = params;
+        p.n_spec = 0;
+        check(vx_trace(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, &p, &lightOut, cmd));
+    }
+};
+
+// LightReflectionPipeline (LightReflectionPipeline.h:12-57): the specular
+// occlusion ray.
+class LightReflectionPipeline {
+public:
+    static LightReflectionPipeline& Get() {
+        static LightReflectionPipeline p;
+        return p;
+    }
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB, vx_volume* shadowVox,
+             const float* palette, const vx_trace_params& params, const vx_trace_out& reflectionOut) {
+        vx_trace_params p = params;
+        p.n_shadow = 0;
+        p.n_ao = 0;
+        p.n_spec = 1;
+        check(vx_trace(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, &p, &reflectionOut, cmd));
+    }
+};
+
+}  // namespace vx

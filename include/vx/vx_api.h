@@ -145,6 +145,18 @@ typedef struct {
     int32_t boxes_grounded;
 } vx_scene_desc;
 
+/* Launch tuning of the trace kernels (no counterpart in the reference, whose
+ * launch shape is one full-screen draw per pass). 0 / -1 fields mean "default";
+ * the autotuner (voxelengine_amd/autotune.py) picks values per workload. */
+typedef struct {
+    int32_t kernel;          /* 0 auto, 1 per-pixel 16x16 tiles, 2 persistent DDA (all-DDA rays only) */
+    int32_t block;           /* persistent kernel threads per block: 256, 512 or 1024 (0 = 512) */
+    int32_t blocks_per_cu;   /* persistent grid = CUs x this (0 = occupancy limit) */
+    int32_t refill;          /* persistent kernel: idle lanes that trigger a refill, 1..64 (0 = 16) */
+    int32_t no_lds_levels;   /* 1: read coarse levels from HBM instead of staging them in LDS */
+    int32_t pad[3];
+} vx_trace_tuning;
+
 /* ---------------------------------------------------------------- context */
 int32_t vx_api_version(void);
 const char* vx_last_error(void);
@@ -152,6 +164,9 @@ const char* vx_last_error(void);
  * HIP device and allocates per-device state. */
 vx_status vx_ctx_create(int32_t device, vx_ctx** out);
 vx_status vx_ctx_destroy(vx_ctx* ctx);
+/* Set (NULL resets) / read the context's trace launch tuning. */
+vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning* tuning);
+vx_status vx_ctx_get_trace_tuning(const vx_ctx* ctx, vx_trace_tuning* tuning);
 
 /* ---------------------------------------------------------------- models
  * VoxAsset (VoxAsset.h:8-62): R8 material volume with 3 mips. Dims are rounded
@@ -213,6 +228,28 @@ vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const
 /* Explicit rays (device, n*7 floats: origin3, dir3, maxt) through one marcher. */
 vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const float* rays_dev, uint32_t n,
                         int32_t march, vx_ray_result* out_dev, void* stream);
+/* ---------------------------------------------------------------- batched trace
+ * Many scenes and views of uneven size in one persistent launch (no
+ * counterpart in the reference, which issues one full-screen draw per pass and
+ * view). All items share the palette and the ray set-up; each has its own
+ * volume, view, G-buffer and outputs. vx_batch_create plans the tile queue,
+ * uploads a device descriptor table and captures the launch in a hipGraph;
+ * vx_batch_launch replays the graph on `stream`. The item buffers must stay
+ * valid while the batch exists. Two launches of one batch must not overlap. */
+typedef struct {
+    const vx_volume* volume;
+    vx_view view;
+    vx_gbuf gbuf;
+    vx_trace_out out;
+} vx_batch_item;
+
+vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette_dev,
+                          const vx_trace_params* params, vx_batch** out);
+vx_status vx_batch_launch(vx_batch* batch, void* stream);
+/* The same work as vx_batch_launch, launched directly instead of through the graph. */
+vx_status vx_batch_launch_direct(vx_batch* batch, void* stream);
+vx_status vx_batch_destroy(vx_batch* batch);
+
 /* Shaded radiance: LightAmbient.frag:178-214 with PBR.frag:3-68, visibility
  * given. light: host {dir3, rgb3}. ssao_dev may be NULL (1.0). rgb_dev: npx*3. */
 vx_status vx_shade_forward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbuf,

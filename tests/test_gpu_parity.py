@@ -201,6 +201,43 @@ def test_trace_pass_matches_oracle(api, ctx, scene64, oracle, march):
     assert 0.0 < sh[hit].mean() < 1.0
 
 
+@pytest.mark.parametrize("jitter", [0, 1])
+def test_trace_pass_crafted_gbuffer(api, ctx, scene64, oracle, jitter):
+    """A hand-made G-buffer with integral positions, zero and axis-aligned normals:
+    rays outside the DDA kernel's fast path (zero or tiny direction components,
+    origins on voxel boundaries) take the deferred exact path; results stay
+    bit-identical to the oracle."""
+    from voxelengine_amd import configs, views
+    model, occ, vol = scene64
+    occ_h = oracle_occ(oracle, vol)
+    w, h = 40, 30
+    n = w * h
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(4.0, 60.0, size=(n, 3)).astype(np.float32)
+    pos[::3] = np.round(pos[::3])
+    nrm = np.zeros((n, 3), np.float32)
+    ax = rng.integers(0, 3, n)
+    nrm[np.arange(n), ax] = rng.choice([-1.0, 1.0], n)
+    nrm[::7] = 0.0
+    nrm[1::11] = rng.normal(size=(len(nrm[1::11]), 3))
+    mat = rng.integers(0, 256, n).astype(np.uint8)
+    mat[::13] = 0
+    vd = views.scene_view(model.dims, w, h)
+    g = api.GBuffer(w, h)
+    g.pos.copy_(torch.from_numpy(pos))
+    g.normal.copy_(torch.from_numpy(nrm))
+    g.material.copy_(torch.from_numpy(mat))
+    pal = configs.make_palette()
+    params = configs.trace_params(n_ao=5, ao_maxt=20.0, march="dda", jitter_shadow=jitter)
+    vis = api.Visibility(w, h, fetches=True)
+    api.trace(ctx, occ, vd, g, torch.from_numpy(pal).cuda(), params, vis)
+    sh, ao, sp, fe = oracle.trace(occ_h, vd, params, pos, nrm, mat, pal)
+    assert_bitwise(vis.shadow.cpu().numpy(), sh, "shadow")
+    assert_bitwise(vis.ao.cpu().numpy(), ao, "ao")
+    assert_bitwise(vis.spec.cpu().numpy(), sp, "spec")
+    assert_bitwise(vis.fetches.cpu().numpy().astype(np.uint32), fe, "fetches")
+
+
 def _shade_inputs(api, ctx, scene64, oracle, w=64, h=48):
     from voxelengine_amd import configs, views
     model, occ, vol = scene64

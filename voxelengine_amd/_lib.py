@@ -69,12 +69,23 @@ class SceneDesc(C.Structure):
                 ("box_min", C.c_int32), ("box_max", C.c_int32), ("boxes_grounded", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("volume", P), ("view", View), ("gbuf", GBuf), ("out", TraceOut)]
+
+
+class TraceTuning(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
+                ("no_lds_levels", C.c_int32), ("pad", C.c_int32 * 3)]
+
+
 # Every symbol the header declares, with its signature.
 SIGNATURES = {
     "vx_api_version": ([], C.c_int32),
     "vx_last_error": ([], C.c_char_p),
     "vx_ctx_create": ([C.c_int32, C.POINTER(P)], C.c_int32),
     "vx_ctx_destroy": ([P], C.c_int32),
+    "vx_ctx_set_trace_tuning": ([P, C.POINTER(TraceTuning)], C.c_int32),
+    "vx_ctx_get_trace_tuning": ([P, C.POINTER(TraceTuning)], C.c_int32),
     "vx_model_create": ([P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(P)], C.c_int32),
     "vx_model_destroy": ([P], C.c_int32),
     "vx_model_dims": ([P, C.POINTER(C.c_int32)], C.c_int32),
@@ -97,6 +108,10 @@ SIGNATURES = {
     "vx_trace": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut), P],
                  C.c_int32),
     "vx_trace_rays": ([P, P, P, C.c_uint32, C.c_int32, P, P], C.c_int32),
+    "vx_batch_create": ([P, C.POINTER(BatchItem), C.c_uint32, P, C.POINTER(TraceParams), C.POINTER(P)], C.c_int32),
+    "vx_batch_launch": ([P, P], C.c_int32),
+    "vx_batch_launch_direct": ([P, P], C.c_int32),
+    "vx_batch_destroy": ([P], C.c_int32),
     "vx_shade_forward": ([P, C.POINTER(View), C.POINTER(GBuf), C.POINTER(TraceOut), P, P, C.POINTER(C.c_float),
                           P, P], C.c_int32),
     "vx_shade_backward": ([P, C.POINTER(View), C.POINTER(GBuf), C.POINTER(TraceOut), P, P,

@@ -41,6 +41,19 @@ class Context:
         self.h = h
         self.device = device
 
+    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, no_lds_levels=0):
+        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, no_lds_levels)
+        check(L.lib().vx_ctx_set_trace_tuning(self.h, C.byref(t)))
+
+    def reset_tuning(self):
+        check(L.lib().vx_ctx_set_trace_tuning(self.h, None))
+
+    def tuning(self) -> dict:
+        t = L.TraceTuning()
+        check(L.lib().vx_ctx_get_trace_tuning(self.h, C.byref(t)))
+        return dict(kernel=t.kernel, block=t.block, blocks_per_cu=t.blocks_per_cu, refill=t.refill,
+                    no_lds_levels=t.no_lds_levels)
+
     def close(self):
         if self.h:
             L.lib().vx_ctx_destroy(self.h)
@@ -248,6 +261,40 @@ def trace_rays(ctx: Context, vol: Volume, rays: torch.Tensor, march: int, stream
     out = torch.zeros((n, RAY_RESULT_DTYPE.itemsize // 4), dtype=torch.int32, device=rays.device)
     check(L.lib().vx_trace_rays(ctx.h, vol.h, _ptr(rays), n, march, _ptr(out), _stream(stream)))
     return out.cpu().numpy().view(RAY_RESULT_DTYPE).reshape(n)
+
+
+class Batch:
+    """vx_batch: many (volume, view, G-buffer, outputs) items in one graph-captured launch."""
+
+    def __init__(self, ctx: Context, items, palette: torch.Tensor, params: dict):
+        """items: list of (Volume, view dict, GBuffer, Visibility)."""
+        arr = (L.BatchItem * len(items))()
+        for k, (vol, vd, g, vis) in enumerate(items):
+            arr[k].volume = vol.h
+            arr[k].view = view_struct(vd)
+            arr[k].gbuf = g.struct()
+            arr[k].out = vis.struct()
+        tp = trace_params(params)
+        h = C.c_void_p()
+        check(L.lib().vx_batch_create(ctx.h, arr, len(items), _ptr(palette), C.byref(tp), C.byref(h)))
+        self.h, self.ctx, self.items, self.palette = h, ctx, items, palette
+
+    def launch(self, stream=None):
+        check(L.lib().vx_batch_launch(self.h, _stream(stream)))
+
+    def launch_direct(self, stream=None):
+        check(L.lib().vx_batch_launch_direct(self.h, _stream(stream)))
+
+    def close(self):
+        if self.h:
+            L.lib().vx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _light(light):

@@ -63,6 +63,10 @@ static vx_status fail(vx_status code, const char* fmt, ...) {
 struct vx_ctx {
     int32_t device;
     long long* grad_acc;     // 256*6+6 fixed-point gradient accumulator
+    unsigned long long* deferred;   // deferred-pixel list of the DDA trace (capacity entries)
+    uint32_t* ndeferred;     // its counter
+    size_t deferred_cap;
+    vx_trace_tuning tuning;
 };
 
 struct vx_model {
@@ -142,10 +146,16 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     if (!c) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
     c->grad_acc = nullptr;
+    c->deferred = nullptr;
+    c->ndeferred = nullptr;
+    c->deferred_cap = 0;
+    std::memset(&c->tuning, 0, sizeof(c->tuning));
     hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
+    if (me == hipSuccess) me = hipMalloc(&c->ndeferred, sizeof(uint32_t));
     if (me != hipSuccess) {
+        (void)hipFree(c->grad_acc);
         delete c;
-        return fail(VX_ERR_OUT_OF_MEMORY, "hipMalloc of the gradient accumulator failed");
+        return fail(VX_ERR_OUT_OF_MEMORY, "context scratch allocation failed");
     }
     *out = c;
     return VX_OK;
@@ -155,7 +165,30 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     if (!ctx) return VX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipFree(ctx->grad_acc);
+    (void)hipFree(ctx->deferred);
+    (void)hipFree(ctx->ndeferred);
     delete ctx;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning* t) {
+    VX_REQUIRE(ctx, "ctx is NULL");
+    if (!t) {
+        std::memset(&ctx->tuning, 0, sizeof(ctx->tuning));
+        return VX_OK;
+    }
+    VX_REQUIRE(t->kernel >= 0 && t->kernel <= 2, "tuning.kernel %d outside 0..2", t->kernel);
+    VX_REQUIRE(t->block == 0 || t->block == 256 || t->block == 512 || t->block == 1024,
+               "tuning.block %d not 256/512/1024", t->block);
+    VX_REQUIRE(t->blocks_per_cu >= 0 && t->blocks_per_cu <= 64, "tuning.blocks_per_cu outside 0..64");
+    VX_REQUIRE(t->refill >= 0 && t->refill <= 64, "tuning.refill outside 0..64");
+    ctx->tuning = *t;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_ctx_get_trace_tuning(const vx_ctx* ctx, vx_trace_tuning* t) {
+    VX_REQUIRE(ctx && t, "ctx/tuning is NULL");
+    *t = ctx->tuning;
     return VX_OK;
 }
 
@@ -652,6 +685,81 @@ extern "C" vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_vie
     return VX_OK;
 }
 
+// Levels staged in LDS: the finest level whose words, with all coarser ones,
+// fit the budget.
+static const size_t kLdsBudget = 48 * 1024;
+
+static void plan_lds(const vx_volume* v, bool stage, DdaTraceArgs& D) {
+    D.lds_first = v->nlevels;
+    D.lds_words = 0;
+    for (int l = 0; l < 3; ++l) D.lds_off[l] = 0;
+    if (!stage) return;
+    for (int first = 0; first < v->nlevels; ++first) {
+        size_t words = 0;
+        for (int l = first; l < v->nlevels; ++l) words += (size_t)v->wx[l] * v->wy[l] * v->wz[l];
+        if (words * 8 <= kLdsBudget) {
+            D.lds_first = first;
+            uint32_t off = 0;
+            for (int l = first; l < v->nlevels; ++l) {
+                D.lds_off[l] = off;
+                off += (uint32_t)((size_t)v->wx[l] * v->wy[l] * v->wz[l]);
+            }
+            D.lds_words = off;
+            break;
+        }
+    }
+}
+
+static vx_status persistent_grid(vx_ctx* ctx, const void* kernel, int block, size_t lds, int want_per_cu,
+                                 uint32_t& grid) {
+    int cus = 0, per_cu = 0;
+    VX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds));
+    if (per_cu < 1) per_cu = 1;
+    if (want_per_cu > 0 && want_per_cu < per_cu) per_cu = want_per_cu;
+    grid = (uint32_t)(cus * per_cu);
+    grid = (grid + 7u) & ~7u;
+    return VX_OK;
+}
+
+// Deferred-pixel list sized to the view (grows; the previous list is released
+// after the device finishes with it).
+static vx_status ensure_deferred(vx_ctx* ctx, size_t npx) {
+    if (ctx->deferred_cap >= npx) return VX_OK;
+    VX_HIP(hipDeviceSynchronize());
+    (void)hipFree(ctx->deferred);
+    ctx->deferred = nullptr;
+    ctx->deferred_cap = 0;
+    VX_HIP(hipMalloc(&ctx->deferred, npx * sizeof(unsigned long long)));
+    ctx->deferred_cap = npx;
+    return VX_OK;
+}
+
+static vx_status launch_trace_dda(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T, hipStream_t stream) {
+    if (vx_status st = ensure_deferred(ctx, (size_t)T.view.width * T.view.height)) return st;
+    const vx_trace_tuning& tu = ctx->tuning;
+    int block = (tu.block == 256 || tu.block == 1024) ? tu.block : 512;
+    DdaTraceArgs D;
+    D.deferred = ctx->deferred;
+    D.ndeferred = ctx->ndeferred;
+    D.t = T;
+    D.tiles_x = cdiv((uint32_t)T.view.width, 8);
+    D.ntiles = D.tiles_x * cdiv((uint32_t)T.view.height, 8);
+    D.band = cdiv(D.ntiles, 8);
+    D.refill = (tu.refill >= 1 && tu.refill <= 64) ? tu.refill : 64;
+    plan_lds(vol, tu.no_lds_levels == 0, D);
+    size_t lds = (size_t)D.lds_words * 8;
+    uint32_t grid = 0;
+    if (vx_status st = persistent_grid(ctx, trace_dda_kernel(block), block, lds, tu.blocks_per_cu, grid)) return st;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ctx->ndeferred, 1);
+    hipError_t e = launch_trace_dda_kernel(block, grid, lds, stream, occ_view(vol), D);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of k_trace_dda failed: %s", hipGetErrorString(e));
+    hipLaunchKernelGGL(k_trace_list, dim3(64), dim3(256), 0, stream, occ_view(vol), T, ctx->deferred,
+                       ctx->ndeferred);
+    VX_LAUNCHED("k_trace_list");
+    return VX_OK;
+}
+
 static vx_status check_params(const vx_trace_params* p) {
     VX_REQUIRE(p, "params is NULL");
     auto okm = [](int32_t m) { return m == VX_MARCH_DDA || m == VX_MARCH_SPARSE || m == VX_MARCH_SUPERSPARSE; };
@@ -679,6 +787,10 @@ extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* 
     A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
     A.palette = palette;
     A.shadow = out->shadow; A.ao = out->ao; A.spec = out->spec; A.fetches = out->fetches;
+    const bool all_dda = (params->n_shadow == 0 || params->shadow_march == VX_MARCH_DDA) &&
+                         (params->n_ao == 0 || params->ao_march == VX_MARCH_DDA) &&
+                         (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA);
+    if (all_dda && ctx->tuning.kernel != 1) return launch_trace_dda(ctx, vol, A, S(stream));
     hipLaunchKernelGGL(k_trace, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
     VX_LAUNCHED("k_trace");
     return VX_OK;
@@ -741,5 +853,148 @@ extern "C" vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const v
     hipLaunchKernelGGL(k_shade_bwd_finish, dim3(cdiv(nacc, 256)), dim3(256), 0, S(stream), ctx->grad_acc, dpalette,
                        dlight);
     VX_LAUNCHED("k_shade_bwd");
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched trace
+// ---------------------------------------------------------------------------
+struct vx_batch {
+    vx_ctx* ctx;
+    uint32_t n;
+    BatchDesc* descs;            // device
+    uint32_t* counters;          // device: 8 band counters + deferred count
+    unsigned long long* deferred;
+    BatchArgs args;
+    int block;
+    uint32_t grid;
+    hipStream_t cap;             // capture stream
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+static void batch_free(vx_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    (void)hipFree(b->descs);
+    (void)hipFree(b->counters);
+    (void)hipFree(b->deferred);
+    delete b;
+}
+
+// The kernels of one batched launch: clear the queue counters, trace, then the
+// deferred pixels.
+static hipError_t batch_enqueue(vx_batch* b, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, b->counters, 9);
+    hipError_t e = launch_trace_batch_kernel(b->block, b->grid, s, b->args);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_trace_list_batch, dim3(64), dim3(256), 0, s, b->descs, b->deferred, b->counters + 8);
+    return hipGetLastError();
+}
+
+extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
+                                     const vx_trace_params* params, vx_batch** out) {
+    VX_REQUIRE(ctx && out && palette, "ctx/out/palette is NULL");
+    VX_REQUIRE(n >= 1 && n <= 65536 && items, "batch needs 1..65536 items");
+    if (vx_status st = check_params(params)) return st;
+    VX_REQUIRE((params->n_shadow == 0 || params->shadow_march == VX_MARCH_DDA) &&
+                   (params->n_ao == 0 || params->ao_march == VX_MARCH_DDA) &&
+                   (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA),
+               "the batched trace runs the DDA marcher for every ray class");
+    *out = nullptr;
+    std::vector<BatchDesc> h(n);
+    uint64_t tiles = 0, pixels = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const vx_batch_item& it = items[i];
+        VX_REQUIRE(it.volume, "item %u: volume is NULL", i);
+        if (vx_status st = check_view(&it.view)) return st;
+        VX_REQUIRE(it.gbuf.pos && it.gbuf.normal && it.gbuf.material && it.out.shadow && it.out.ao && it.out.spec,
+                   "item %u: NULL buffer", i);
+        BatchDesc& D = h[i];
+        std::memset(&D, 0, sizeof(D));
+        D.V = occ_view(it.volume);
+        D.T.view = it.view;
+        D.T.params = *params;
+        D.T.tiles = tile_map(it.view.width, it.view.height);
+        D.T.pos = it.gbuf.pos; D.T.nrm = it.gbuf.normal; D.T.mat = it.gbuf.material;
+        D.T.palette = palette;
+        D.T.shadow = it.out.shadow; D.T.ao = it.out.ao; D.T.spec = it.out.spec; D.T.fetches = it.out.fetches;
+        D.tiles_x = cdiv((uint32_t)it.view.width, 8);
+        D.ntiles = D.tiles_x * cdiv((uint32_t)it.view.height, 8);
+        D.tile0 = (uint32_t)tiles;
+        tiles += D.ntiles;
+        pixels += (uint64_t)it.view.width * it.view.height;
+        VX_REQUIRE(tiles < (1ull << 31), "batch too large");
+    }
+    if (vx_status st = use_device(ctx)) return st;
+    vx_batch* b = new (std::nothrow) vx_batch();
+    if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    std::memset(b, 0, sizeof(*b));
+    b->ctx = ctx;
+    b->n = n;
+    hipError_t e = hipMalloc(&b->descs, n * sizeof(BatchDesc));
+    if (e == hipSuccess) e = hipMalloc(&b->counters, 9 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->deferred, pixels * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemcpy(b->descs, h.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        batch_free(b);
+        return fail(VX_ERR_OUT_OF_MEMORY, "batch allocation failed: %s", hipGetErrorString(e));
+    }
+    const vx_trace_tuning& tu = ctx->tuning;
+    b->block = (tu.block == 256 || tu.block == 1024) ? tu.block : 512;
+    BatchArgs& B = b->args;
+    B.descs = b->descs;
+    B.nitems = n;
+    B.ntiles = (uint32_t)tiles;
+    B.band = cdiv(tiles, 8);
+    B.counters = b->counters;
+    B.deferred = b->deferred;
+    B.ndeferred = b->counters + 8;
+    B.refill = (tu.refill >= 1 && tu.refill <= 64) ? tu.refill : 64;
+    if (vx_status st = persistent_grid(ctx, trace_batch_kernel(b->block), b->block, 0, tu.blocks_per_cu, b->grid)) {
+        batch_free(b);
+        return st;
+    }
+    // Capture the launch once; vx_batch_launch replays it.
+    e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = batch_enqueue(b, b->cap);
+        e = hipStreamEndCapture(b->cap, &b->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        batch_free(b);
+        return fail(VX_ERR_DEVICE, "batch graph capture failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_batch_launch(vx_batch* b, void* stream) {
+    VX_REQUIRE(b, "batch is NULL");
+    if (vx_status st = use_device(b->ctx)) return st;
+    VX_HIP(hipGraphLaunch(b->exec, S(stream)));
+    return VX_OK;
+}
+
+extern "C" vx_status vx_batch_launch_direct(vx_batch* b, void* stream) {
+    VX_REQUIRE(b, "batch is NULL");
+    if (vx_status st = use_device(b->ctx)) return st;
+    hipError_t e = batch_enqueue(b, S(stream));
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+extern "C" vx_status vx_batch_destroy(vx_batch* b) {
+    if (!b) return VX_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    batch_free(b);
     return VX_OK;
 }

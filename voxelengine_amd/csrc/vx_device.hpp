@@ -102,12 +102,13 @@ VXD void tangent_frame(f3 n, f3& t, f3& b) {
 
 // ---------------------------------------------------------------------------
 // Occupancy volume in brick layout.
-//   fine voxel f -> texel f>>1 -> cell f>>2 (u64 = 2x2x2 texels) -> brick f>>3
-//   (64 B = 2x2x2 cells). Bricks X-fastest; cell c within a brick and texel t
-//   within a cell are (x&1)|(y&1)<<1|(z&1)<<2 of the respective coordinates;
-//   bit b within the texel byte is the reference's (ShadowVoxSystem.cpp:85).
-//   Brick grid is padded by one so texel index s (the first out-of-range one,
-//   reachable by the DDA bound 2*s+1) stays inside and reads 0.
+//   fine voxel f -> cell f>>2 (u64 = 4x4x4 fine voxels, bit (x&3)|(y&3)<<2|
+//   (z&3)<<4) -> brick f>>3 (64 B = 2x2x2 cells, cell (cx&1)|(cy&1)<<1|(cz&1)<<2).
+//   Bricks X-fastest. A reference texel byte (2x2x2 fine voxels, bit
+//   (x&1)|(y&1)<<1|(z&1)<<2, ShadowVoxSystem.cpp:85) occupies cell bits
+//   base + {0,1,4,5,16,17,20,21} with base = 2(tx&1) + 8(ty&1) + 32(tz&1).
+//   The brick grid is padded by one so texel index s (the first out-of-range
+//   one, reachable by the DDA bound 2*s+1) stays inside and reads 0.
 // Coarse level k (1..3): 1 bit per (8^k)^3 fine voxels, u64 words of 4x4x4
 // bits, bit (cx&3)|(cy&3)<<2|(cz&3)<<4, words X-fastest.
 // ---------------------------------------------------------------------------
@@ -132,13 +133,22 @@ VXD size_t cell_index(const OccView& V, int32_t fx, int32_t fy, int32_t fz) {
 }
 // Bit of fine voxel f inside its cell word.
 VXD uint32_t cell_bit(int32_t fx, int32_t fy, int32_t fz) {
-    return sub3(fx >> 1, fy >> 1, fz >> 1) * 8u + sub3(fx, fy, fz);
+    return (uint32_t)((fx & 3) | ((fy & 3) << 2) | ((fz & 3) << 4));
+}
+// Cell-bit offset of texel t's byte (see layout above).
+VXD uint32_t texel_base(int32_t tx, int32_t ty, int32_t tz) {
+    return (uint32_t)(((tx & 1) << 1) | ((ty & 1) << 3) | ((tz & 1) << 5));
+}
+// Spread a reference texel byte to its cell-bit pattern (relative to texel_base).
+VXD uint32_t spread_byte(uint32_t b) {
+    return (b & 3u) | ((b & 0xcu) << 2) | ((b & 0x30u) << 12) | ((b & 0xc0u) << 14);
 }
 // Reference byte of texel t (0 when outside [0, s)), texelFetch semantics.
 VXD uint32_t texel_byte(const OccView& V, int32_t tx, int32_t ty, int32_t tz) {
     if ((uint32_t)tx >= (uint32_t)V.sx || (uint32_t)ty >= (uint32_t)V.sy || (uint32_t)tz >= (uint32_t)V.sz) return 0u;
     uint64_t w = V.cells[brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1)];
-    return (uint32_t)(w >> (sub3(tx, ty, tz) * 8u)) & 0xffu;
+    uint32_t x = (uint32_t)(w >> texel_base(tx, ty, tz));
+    return (x & 3u) | ((x >> 2) & 0xcu) | ((x >> 12) & 0x30u) | ((x >> 14) & 0xc0u);
 }
 VXD bool level_bit(const OccView& V, int32_t lvl, int32_t cx, int32_t cy, int32_t cz) {
     size_t wi = ((size_t)(cz >> 2) * V.wy[lvl] + (cy >> 2)) * V.wx[lvl] + (cx >> 2);

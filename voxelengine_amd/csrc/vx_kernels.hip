@@ -144,13 +144,12 @@ __global__ void k_bytes_to_cells(uint64_t* cells, const uint8_t* bytes, OccView 
     int32_t tz = rz + (int32_t)(i / ((size_t)rw * rh));
     uint8_t b = bytes[(size_t)tx + (size_t)ty * V.sx + (size_t)tz * V.sx * V.sy];
     size_t ci = brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1);
-    uint32_t sh = sub3(tx, ty, tz) * 8u;
-    // Eight texels share a cell word: update the byte with a read-modify-write
-    // on the 32-bit half that holds it.
-    uint32_t* w32 = reinterpret_cast<uint32_t*>(cells + ci) + (sh >> 5);
-    uint32_t s32 = sh & 31u;
-    atomicAnd(w32, ~(0xffu << s32));
-    atomicOr(w32, (uint32_t)b << s32);
+    uint32_t base = texel_base(tx, ty, tz);
+    // Eight texels share a cell word; each texel's bits lie in one 32-bit half.
+    uint32_t* w32 = reinterpret_cast<uint32_t*>(cells + ci) + (base >> 5);
+    uint32_t s32 = base & 31u;
+    atomicAnd(w32, ~(0x330033u << s32));
+    atomicOr(w32, spread_byte(b) << s32);
 }
 
 // Brick layout -> reference byte layout (one thread per texel, whole volume).
@@ -444,12 +443,25 @@ VXD bool tile_pixel(const TileMap& tm, int32_t& px, int32_t& py) {
 
 // Ambient-pass pixel (LightAmbient.frag:134-175) generalised to n_ao cosine AO
 // rays with a length cap, plus a GGX-sampled specular-occlusion ray
-// (LightReflection.frag:86-113 ray origin).
+// (LightReflection.frag:86-113 ray origin), with the literal marchers.
+VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i);
+
 __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A) {
     int32_t px, py;
     if (!tile_pixel(A.tiles, px, py)) return;
+    trace_pixel_exact(V, A, (size_t)py * A.view.width + px);
+}
+
+// Pixels deferred by k_trace_dda (a ray outside its fast path), traced whole.
+__global__ void __launch_bounds__(256) k_trace_list(OccView V, TraceArgs A, const unsigned long long* list,
+                                                    const uint32_t* count) {
+    uint32_t n = *count;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        trace_pixel_exact(V, A, (uint32_t)list[j]);
+}
+
+VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
     const vx_trace_params& P = A.params;
-    size_t i = (size_t)py * A.view.width + px;
     uint8_t m = A.mat[i];
     if (m == 0) {
         A.shadow[i] = 0; A.ao[i] = 0.0f; A.spec[i] = 0.0f;
@@ -543,6 +555,492 @@ __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A) {
     }
     A.spec[i] = spv;
     if (A.fetches) A.fetches[i] = nf;
+}
+
+// ===========================================================================
+// Persistent DDA trace: every ray class through raycastShadowVolume semantics
+// (Light.frag:29-81), one ray per lane.
+//
+// Fast path. For a ray with all |dir| >= 2^-100 (finite), |origin| < 2^22 and
+// no negative-direction axis starting exactly on a voxel boundary, every
+// t_max is positive and finite for the whole march. Then, exactly:
+//   select_k = (t_max_k == min3(t_max))          (step(): no NaNs)
+//   best_t   = dot(t_max, select) = t, t+t or (t+t)+t for 1, 2 or 3 ties
+//   t_max_k += t_delta_k * select_k  ==  select_k ? t_max_k + t_delta_k : t_max_k
+//   current_voxel_k += ivec3(select_k * sign_k) == select_k ? sign_k : 0
+// Pixels with any other ray are traced whole by k_trace_list (literal path).
+// ===========================================================================
+
+// Per-launch constants of the tile tracer: staged levels, the deferred-pixel
+// list (entries item << 32 | pixel) and the refill threshold.
+struct TileCtx {
+    const uint64_t* s_lvl;
+    int32_t lds_first;
+    uint32_t lds_off[3];
+    unsigned long long* deferred;
+    uint32_t* ndeferred;
+    uint32_t item;
+    int32_t refill;
+};
+
+// A prepared ray: DDA state at the origin (t_max, t_delta, voxel), step signs
+// packed in bits 0..2 (1 = negative), kind in bits 4..5.
+struct PendRay {
+    float tm0, tm1, tm2, td0, td1, td2;
+    int32_t cx, cy, cz;
+    uint32_t meta;
+    float maxt;
+};
+
+// The marching ray. ckey is the packed 4^3-cell coordinate whose occupancy
+// word is held in cw (0 for cells of an empty brick); bkey the packed brick
+// coordinate of the last coarse-level lookup, bempty its result.
+struct LaneRay {
+    float tm0, tm1, tm2, td0, td1, td2;
+    int32_t cx, cy, cz;
+    int32_t stx, sty, stz;
+    int32_t n;
+    float maxt;
+    int32_t bkey;
+    int32_t ckey;
+    bool bempty;
+    uint64_t cw;
+};
+
+VXD bool fast_ray_ok(f3 o, f3 d) {
+    const float lo = 7.8886091e-31f;   // 2^-100
+    bool ok = true;
+    float dd[3] = {d.x, d.y, d.z}, oo[3] = {o.x, o.y, o.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float a = fabsf(dd[k]);
+        ok = ok && (a >= lo) && (a <= 3.0e38f) && (fabsf(oo[k]) < 4194304.0f);
+        ok = ok && !(dd[k] < 0.0f && floorf(oo[k]) == oo[k]);
+    }
+    return ok;
+}
+
+// DDA state at the origin (Light.frag:32-46); |o| < 2^22 so the conversions
+// are exact without saturation.
+VXD void prepare_ray(PendRay& P, f3 o, f3 d, float maxt, int32_t kind) {
+    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, tm[3], td[3];
+    int32_t cv[3];
+    uint32_t meta = (uint32_t)kind << 4;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        bool neg = !(dd[k] > 0.0f);
+        float s = neg ? -1.0f : 1.0f;
+        meta |= neg ? (1u << k) : 0u;
+        td[k] = 1.0f / (dd[k] * s);
+        float fl = floorf(oo[k]);
+        cv[k] = (int32_t)fl;
+        float nb = fl + (neg ? 0.0f : 1.0f);      // vec3(cv) + (sign*0.5 + 0.5)
+        tm[k] = (nb - oo[k]) / dd[k];
+    }
+    P.tm0 = tm[0]; P.tm1 = tm[1]; P.tm2 = tm[2];
+    P.td0 = td[0]; P.td1 = td[1]; P.td2 = td[2];
+    P.cx = cv[0]; P.cy = cv[1]; P.cz = cv[2];
+    P.meta = meta;
+    P.maxt = maxt;
+}
+
+VXD int32_t begin_ray(LaneRay& R, const PendRay& P) {
+    R.tm0 = P.tm0; R.tm1 = P.tm1; R.tm2 = P.tm2;
+    R.td0 = P.td0; R.td1 = P.td1; R.td2 = P.td2;
+    R.cx = P.cx; R.cy = P.cy; R.cz = P.cz;
+    R.stx = (P.meta & 1u) ? -1 : 1;
+    R.sty = (P.meta & 2u) ? -1 : 1;
+    R.stz = (P.meta & 4u) ? -1 : 1;
+    R.n = 0;
+    R.maxt = P.maxt;
+    R.bkey = -1;   // brick keys are < 2^31
+    R.ckey = ((P.cx >> 2) | ((P.cy >> 2) << 11) | ((P.cz >> 2) << 22)) ^ 1;   // differs from the first cell
+    R.bempty = true;
+    R.cw = 0;
+    return (int32_t)((P.meta >> 4) & 3u);
+}
+
+VXD bool brick_empty_staged(const OccView& V, const TileCtx& A, int32_t bx, int32_t by, int32_t bz) {
+    const uint64_t* s_lvl = A.s_lvl;
+    if (V.nlevels == 0) return false;
+    if (A.lds_first == 0) {
+        uint32_t wi = __umul24(__umul24((uint32_t)(bz >> 2), (uint32_t)V.wy[0]) + (uint32_t)(by >> 2), (uint32_t)V.wx[0]) +
+                      (uint32_t)(bx >> 2);
+        uint32_t b = (uint32_t)((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
+        return !((s_lvl[A.lds_off[0] + wi] >> b) & 1ull);
+    }
+    for (int32_t l = V.nlevels - 1; l >= 0; --l) {
+        int32_t sh = 3 * l;
+        int32_t cx = bx >> sh, cy = by >> sh, cz = bz >> sh;
+        uint32_t wi = __umul24(__umul24((uint32_t)(cz >> 2), (uint32_t)V.wy[l]) + (uint32_t)(cy >> 2), (uint32_t)V.wx[l]) +
+                      (uint32_t)(cx >> 2);
+        uint32_t b = (uint32_t)((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
+        uint64_t w = (l >= A.lds_first) ? s_lvl[A.lds_off[l] + wi] : V.lvl[l][wi];
+        if (!((w >> b) & 1ull)) return true;
+    }
+    return false;
+}
+
+// One DDA iteration. Returns 0 to continue, 1 hit, 2 out of bounds, 3 loop end
+// (n == 256 or totalt >= maxt: the x4 restart repeats this pass).
+VXD int32_t dda_step(const OccView& V, const TileCtx& A, LaneRay& R, float& best_t) {
+    float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
+    bool s0 = R.tm0 == tmin, s1 = R.tm1 == tmin, s2 = R.tm2 == tmin;
+    if ((uint32_t)R.cx > (uint32_t)(2 * V.sx + 1) || (uint32_t)R.cy > (uint32_t)(2 * V.sy + 1) ||
+        (uint32_t)R.cz > (uint32_t)(2 * V.sz + 1))
+        return 2;
+    // cell coordinates <= (2*4095+1)>>2 = 2047 fit 11 bits each
+    int32_t ckey = (R.cx >> 2) | ((R.cy >> 2) << 11) | ((R.cz >> 2) << 22);
+    if (ckey != R.ckey) {
+        R.ckey = ckey;
+        int32_t bx = R.cx >> 3, by = R.cy >> 3, bz = R.cz >> 3;
+        int32_t bkey = bx | (by << 10) | (bz << 20);
+        if (bkey != R.bkey) {
+            R.bkey = bkey;
+            R.bempty = brick_empty_staged(V, A, bx, by, bz);
+        }
+        uint64_t w = 0;
+        if (!R.bempty) {
+            uint32_t bi = __umul24(__umul24((uint32_t)bz, (uint32_t)V.by) + (uint32_t)by, (uint32_t)V.bx) + (uint32_t)bx;
+            w = V.cells[(size_t)bi * 8 + sub3(R.cx >> 2, R.cy >> 2, R.cz >> 2)];
+        }
+        R.cw = w;
+    }
+    bool vox = (R.cw >> cell_bit(R.cx, R.cy, R.cz)) & 1ull;
+    best_t = tmin;
+    bool tie = (s0 && s1) || (s0 && s2) || (s1 && s2);
+    if (tie) best_t = (s0 && s1 && s2) ? ((tmin + tmin) + tmin) : (tmin + tmin);
+    if (vox) return 1;
+    R.cx = s0 ? R.cx + R.stx : R.cx;
+    R.cy = s1 ? R.cy + R.sty : R.cy;
+    R.cz = s2 ? R.cz + R.stz : R.cz;
+    R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
+    R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
+    R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
+    R.n++;
+    if (!(R.n < 256 && best_t < R.maxt)) return 3;
+    return 0;
+}
+
+// Per-lane pixel state; the G-buffer is re-read where a ray set-up needs it.
+struct PixState {
+    f3 org;
+    uint32_t pk;
+    uint32_t i;
+    int32_t r;
+    uint32_t nf;
+    float aosum, spv;
+    uint32_t lit;
+};
+
+enum { RAY_SHADOW = 0, RAY_AO = 1, RAY_SPEC = 2 };
+
+// Pixel set-up shared by its rays (LightAmbient.frag:136-158). Returns false
+// for sky pixels (outputs written). wd receives the shadow direction.
+VXD bool pixel_init(const TraceArgs& A, uint32_t i, PixState& S, f3& wd) {
+    const vx_trace_params& P = A.params;
+    uint32_t m = A.mat[i];
+    if (m == 0) {
+        A.shadow[i] = 0; A.ao[i] = 0.0f; A.spec[i] = 0.0f;
+        if (A.fetches) A.fetches[i] = 0;
+        return false;
+    }
+    f3 p = load3(A.pos + 3 * (size_t)i);
+    f3 n = load3(A.nrm + 3 * (size_t)i);
+    S.i = i;
+    S.pk = pixkey(P.seed, P.frame, i);
+    S.r = 0;
+    S.nf = 0;
+    S.aosum = 0.0f;
+    S.spv = 1.0f;
+    S.lit = 1;
+    float r0[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r0[k] = rand01(S.pk, (uint32_t)k);
+    float wx = p.x * 0.1f, wy = p.y * 0.1f, wz = p.z * 0.1f;
+    const float* Vm = A.view.view;
+    float vz = ((Vm[2] * wx + Vm[6] * wy) + Vm[10] * wz) + Vm[14];
+    float depth = ((-vz) - 0.1f) / (4096.0f - 0.1f);
+    float st = gclamp((depth - 0.0f) / (0.2f - 0.0f), 0.0f, 1.0f);
+    float bias = (st * st) * (3.0f - 2.0f * st) * 50.0f + 1.5f;
+    wd = mk3(P.light_dir[0], P.light_dir[1], P.light_dir[2]);
+    f3 wcp = p;
+    if (P.jitter_shadow) {
+        f3 rv = cosine_hemisphere(r0[0], r0[1]);
+        rv = mk3(rv.x * 0.1f, rv.y * 0.1f, rv.z * 0.1f);
+        rv.z = rv.z * gsign(r0[2] - 0.5f);
+        f3 mx = mk3(wd.x * (1.0f - 0.5f) + rv.x * 0.5f, wd.y * (1.0f - 0.5f) + rv.y * 0.5f,
+                    wd.z * (1.0f - 0.5f) + rv.z * 0.5f);
+        wd = normalize3(mx);
+        float w = r0[3] * 1.0f;
+        wcp = mk3(wcp.x + wd.x * w, wcp.y + wd.y * w, wcp.z + wd.z * w);
+        wcp = mk3(wcp.x + rv.x * 2.5f, wcp.y + rv.y * 2.5f, wcp.z + rv.z * 2.5f);
+    }
+    S.org = mk3(wcp.x + n.x * bias, wcp.y + n.y * bias, wcp.z + n.z * bias);
+    return true;
+}
+
+// Origin, direction and maxt of ray r >= n_shadow of the pixel (AO or spec).
+VXD int32_t ray_setup(const TraceArgs& A, const PixState& S, int32_t r, f3& o, f3& d, float& maxt) {
+    const vx_trace_params& P = A.params;
+    f3 n = load3(A.nrm + 3 * (size_t)S.i);
+    f3 T, B;
+    tangent_frame(n, T, B);
+    int32_t a = r - P.n_shadow;
+    if (a < P.n_ao) {
+        float u1 = rand01(S.pk, (uint32_t)(4 * (1 + a) + 0));
+        float u2 = rand01(S.pk, (uint32_t)(4 * (1 + a) + 1));
+        f3 rv = cosine_hemisphere(u1, u2);
+        d = mk3((T.x * rv.x + B.x * rv.y) + n.x * rv.z, (T.y * rv.x + B.y * rv.y) + n.y * rv.z,
+                (T.z * rv.x + B.z * rv.y) + n.z * rv.z);
+        o = S.org;
+        maxt = P.ao_maxt;
+        return RAY_AO;
+    }
+    f3 p = load3(A.pos + 3 * (size_t)S.i);
+    uint32_t base = (uint32_t)(4 * (1 + P.n_ao));
+    float u1 = rand01(S.pk, base + 0), u2 = rand01(S.pk, base + 1), u3 = rand01(S.pk, base + 3);
+    float rough = A.palette[6 * (uint32_t)A.mat[S.i] + 3];
+    f3 cam = mk3(A.view.cam_pos[0] * 10.0f, A.view.cam_pos[1] * 10.0f, A.view.cam_pos[2] * 10.0f);
+    f3 Vd = normalize3(mk3(cam.x - p.x, cam.y - p.y, cam.z - p.z));
+    float alpha = rough * rough;
+    float a2 = alpha * alpha;
+    float ct2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
+    float ct = sqrtf(ct2);
+    float stt = sqrtf(gmax(0.0f, 1.0f - ct2));
+    float sp, cp;
+    sincos_vx(6.2831855f * u2, &sp, &cp);
+    f3 hl = mk3(stt * cp, stt * sp, ct);
+    f3 h = mk3((T.x * hl.x + B.x * hl.y) + n.x * hl.z, (T.y * hl.x + B.y * hl.y) + n.y * hl.z,
+               (T.z * hl.x + B.z * hl.y) + n.z * hl.z);
+    f3 I = mk3(-Vd.x, -Vd.y, -Vd.z);
+    float dhi = dot3(h, I);
+    d = mk3(I.x - (2.0f * dhi) * h.x, I.y - (2.0f * dhi) * h.y, I.z - (2.0f * dhi) * h.z);
+    o = mk3((p.x + n.x * u3) + n.x, (p.y + n.y * u3) + n.y, (p.z + n.z * u3) + n.z);
+    maxt = P.spec_maxt;
+    return RAY_SPEC;
+}
+
+VXD void ray_finish(PixState& S, int32_t kind, int32_t hit, int32_t term, int32_t nt, float t, float maxt) {
+    S.nf += (uint32_t)(term == 0 ? 4 * nt : nt + (term == 1 ? 1 : 0));
+    if (kind == RAY_SHADOW) {
+        S.lit = hit ? 0u : 1u;
+        return;
+    }
+    float dist = hit ? gmin(t, maxt) : maxt;
+    float q = gmin(dist, maxt) / maxt;
+    if (kind == RAY_AO) S.aosum = S.aosum + q * q;
+    else S.spv = q;
+}
+
+VXD void pixel_write(const TraceArgs& A, const PixState& S) {
+    const vx_trace_params& P = A.params;
+    A.shadow[S.i] = (uint8_t)S.lit;
+    A.ao[S.i] = P.n_ao > 0 ? S.aosum / (float)P.n_ao : 1.0f;
+    A.spec[S.i] = P.n_spec > 0 ? S.spv : 1.0f;
+    if (A.fetches) A.fetches[S.i] = S.nf;
+}
+
+// Prepares the pixel's next ray into P; returns false when that ray is outside
+// the fast path (the pixel is then deferred to the literal path).
+VXD bool prepare_next(const TraceArgs& T, const TileCtx& C, PixState& S, PendRay& P, f3 wd) {
+    f3 o, d;
+    float maxt;
+    int32_t kind;
+    if (S.r < T.params.n_shadow) {
+        o = S.org; d = wd; maxt = T.params.shadow_maxt;
+        kind = RAY_SHADOW;
+    } else {
+        kind = ray_setup(T, S, S.r, o, d, maxt);
+    }
+    S.r++;
+    if (fast_ray_ok(o, d)) {
+        prepare_ray(P, o, d, maxt, kind);
+        return true;
+    }
+    uint32_t slot = atomicAdd(C.ndeferred, 1u);
+    C.deferred[slot] = ((unsigned long long)C.item << 32) | (unsigned long long)S.i;
+    return false;
+}
+
+// One 8x8-pixel tile per wave; lane l owns pixel l and runs its rays in order.
+// A lane whose ray ends starts its next ray once `refill` lanes wait for one
+// (or no lane is marching); refill 64 keeps the rays of a tile in lockstep.
+VXD void trace_tile(const OccView& V, const TraceArgs& T, const TileCtx& C, uint32_t px, uint32_t py) {
+    const int32_t nrays = T.params.n_shadow + T.params.n_ao + T.params.n_spec;
+    PixState S;
+    LaneRay R;
+    int32_t kind = RAY_SHADOW;
+    bool valid = false, active = false;
+    if (px < (uint32_t)T.view.width && py < (uint32_t)T.view.height) {
+        f3 wd;
+        valid = pixel_init(T, py * (uint32_t)T.view.width + px, S, wd);
+        if (valid && nrays > 0) {
+            PendRay P;
+            valid = prepare_next(T, C, S, P, wd);
+            if (valid) {
+                kind = begin_ray(R, P);
+                active = true;
+            }
+        } else if (valid) {
+            pixel_write(T, S);
+            valid = false;
+        }
+    }
+    while (true) {
+        bool need = valid && !active;
+        uint64_t needm = __ballot(need);
+        uint64_t actm = __ballot(active);
+        if (needm == 0 && actm == 0) break;
+        if (needm != 0 && (actm == 0 || __popcll(needm) >= (uint32_t)C.refill)) {
+            if (need) {
+                PendRay P;
+                if (prepare_next(T, C, S, P, mk3(0.0f, 0.0f, 0.0f))) {
+                    kind = begin_ray(R, P);
+                    active = true;
+                } else {
+                    valid = false;
+                }
+            }
+        }
+        int32_t res = 0;
+        float bt = 0.0f;
+#pragma unroll
+        for (int32_t u = 0; u < 4; ++u) {
+            if (active && res == 0) {
+                float best_t;
+                res = dda_step(V, C, R, best_t);
+                bt = best_t;
+            }
+        }
+        if (res != 0) {
+            ray_finish(S, kind, res == 1, res == 1 ? 1 : (res == 2 ? 2 : 0), R.n, bt, R.maxt);
+            active = false;
+            if (S.r >= nrays) {
+                pixel_write(T, S);
+                valid = false;
+            }
+        }
+    }
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_dda(OccView V, DdaTraceArgs A) {
+    extern __shared__ uint64_t s_lvl[];
+    for (int32_t l = A.lds_first; l < V.nlevels; ++l) {
+        uint32_t nw = (uint32_t)V.wx[l] * (uint32_t)V.wy[l] * (uint32_t)V.wz[l];
+        for (uint32_t j = threadIdx.x; j < nw; j += BLOCK) s_lvl[A.lds_off[l] + j] = V.lvl[l][j];
+    }
+    __syncthreads();
+    TileCtx C;
+    C.s_lvl = s_lvl;
+    C.lds_first = A.lds_first;
+    C.lds_off[0] = A.lds_off[0]; C.lds_off[1] = A.lds_off[1]; C.lds_off[2] = A.lds_off[2];
+    C.deferred = A.deferred;
+    C.ndeferred = A.ndeferred;
+    C.item = 0;
+    C.refill = A.refill;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t xcd = blockIdx.x & 7u;
+    const uint32_t wave = (blockIdx.x >> 3) * (BLOCK / 64) + (threadIdx.x >> 6);   // wave index within the XCD
+    const uint32_t nwave = (gridDim.x >> 3) * (BLOCK / 64);                         // waves per XCD
+    const uint32_t band0 = xcd * A.band;
+    const uint32_t band1 = min(A.ntiles, band0 + A.band);
+    // The waves of an XCD walk the tiles of its band.
+    for (uint32_t tile = band0 + wave; tile < band1; tile += nwave)
+        trace_tile(V, A.t, C, (tile % A.tiles_x) * 8 + (lane & 7u), (tile / A.tiles_x) * 8 + (lane >> 3));
+}
+
+// Batched trace: many scenes and views in one launch. The tiles of all items
+// form one sequence cut into 8 contiguous bands, one per XCD; a wave takes the
+// next tile of its XCD's band from an atomic counter and moves on to the
+// other bands once its own is exhausted.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_batch(BatchArgs B) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t q = blockIdx.x & 7u;
+    uint32_t misses = 0;
+    while (misses < 8) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&B.counters[q], 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        uint32_t tile = q * B.band + t;
+        if (t >= B.band || tile >= B.ntiles) {
+            q = (q + 1u) & 7u;
+            misses++;
+            continue;
+        }
+        misses = 0;
+        uint32_t lo = 0, hi = B.nitems - 1;          // item with tile0 <= tile
+        while (lo < hi) {
+            uint32_t mid = (lo + hi + 1u) >> 1;
+            if (B.descs[mid].tile0 <= tile) lo = mid;
+            else hi = mid - 1u;
+        }
+        const uint32_t item = __builtin_amdgcn_readfirstlane(lo);
+        const BatchDesc& D = B.descs[item];
+        TileCtx C;
+        C.s_lvl = nullptr;
+        C.lds_first = D.V.nlevels;
+        C.lds_off[0] = C.lds_off[1] = C.lds_off[2] = 0;
+        C.deferred = B.deferred;
+        C.ndeferred = B.ndeferred;
+        C.item = item;
+        C.refill = B.refill;
+        uint32_t lt = tile - D.tile0;
+        trace_tile(D.V, D.T, C, (lt % D.tiles_x) * 8 + (lane & 7u), (lt / D.tiles_x) * 8 + (lane >> 3));
+    }
+}
+
+// Deferred pixels of a batch, traced whole by the literal path.
+__global__ void __launch_bounds__(256) k_trace_list_batch(const BatchDesc* descs, const unsigned long long* list,
+                                                          const uint32_t* count) {
+    uint32_t n = *count;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        unsigned long long e = list[j];
+        const BatchDesc& D = descs[(uint32_t)(e >> 32)];
+        trace_pixel_exact(D.V, D.T, (uint32_t)e);
+    }
+}
+
+template <int BLOCK>
+static const void* dda_kernel() {
+    return reinterpret_cast<const void*>(&k_trace_dda<BLOCK>);
+}
+
+const void* trace_dda_kernel(int block) {
+    switch (block) {
+        case 256: return dda_kernel<256>();
+        case 1024: return dda_kernel<1024>();
+        default: return dda_kernel<512>();
+    }
+}
+
+hipError_t launch_trace_dda_kernel(int block, uint32_t grid, size_t lds, hipStream_t stream, const OccView& V,
+                                   const DdaTraceArgs& A) {
+    switch (block) {
+        case 256: hipLaunchKernelGGL(k_trace_dda<256>, dim3(grid), dim3(256), lds, stream, V, A); break;
+        case 1024: hipLaunchKernelGGL(k_trace_dda<1024>, dim3(grid), dim3(1024), lds, stream, V, A); break;
+        default: hipLaunchKernelGGL(k_trace_dda<512>, dim3(grid), dim3(512), lds, stream, V, A); break;
+    }
+    return hipGetLastError();
+}
+
+const void* trace_batch_kernel(int block) {
+    switch (block) {
+        case 256: return reinterpret_cast<const void*>(&k_trace_batch<256>);
+        case 1024: return reinterpret_cast<const void*>(&k_trace_batch<1024>);
+        default: return reinterpret_cast<const void*>(&k_trace_batch<512>);
+    }
+}
+
+hipError_t launch_trace_batch_kernel(int block, uint32_t grid, hipStream_t stream, const BatchArgs& B) {
+    switch (block) {
+        case 256: hipLaunchKernelGGL(k_trace_batch<256>, dim3(grid), dim3(256), 0, stream, B); break;
+        case 1024: hipLaunchKernelGGL(k_trace_batch<1024>, dim3(grid), dim3(1024), 0, stream, B); break;
+        default: hipLaunchKernelGGL(k_trace_batch<512>, dim3(grid), dim3(512), 0, stream, B); break;
+    }
+    return hipGetLastError();
 }
 
 // ===========================================================================
@@ -814,6 +1312,11 @@ __global__ void k_shade_bwd_finish(const long long* acc, float* dpal, float* dli
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < 256 * 6) dpal[j] = (float)((double)acc[j] * (1.0 / 1073741824.0));
     else if (j < 256 * 6 + 6) dlight[j - 256 * 6] = (float)((double)acc[j] * (1.0 / 1073741824.0));
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) p[j] = 0;
 }
 
 __global__ void k_zero_i64(long long* p, int n) {
