@@ -149,11 +149,11 @@ typedef struct {
  * launch shape is one full-screen draw per pass). 0 / -1 fields mean "default";
  * the autotuner (voxelengine_amd/autotune.py) picks values per workload. */
 typedef struct {
-    int32_t kernel;          /* 0 auto, 1 per-pixel 16x16 tiles, 2 persistent DDA (all-DDA rays only) */
-    int32_t block;           /* persistent kernel threads per block: 256, 512 or 1024 (0 = 512) */
+    int32_t kernel;          /* 0 auto, 1 per-pixel 16x16 tiles, 2 ray-parallel DDA (all-DDA rays only) */
+    int32_t block;           /* ray-parallel kernel threads per block: 256 or 512 (0 = 256) */
     int32_t blocks_per_cu;   /* persistent grid = CUs x this (0 = occupancy limit) */
-    int32_t refill;          /* persistent kernel: idle lanes that trigger a refill, 1..64 (0 = 16) */
-    int32_t no_lds_levels;   /* 1: read coarse levels from HBM instead of staging them in LDS */
+    int32_t refill;          /* lanes without a running ray that trigger a refill, 1..64 (0 = 16) */
+    int32_t steps;           /* DDA steps between refill checks: 2, 4 or 8 (0 = 4) */
     int32_t pad[3];
 } vx_trace_tuning;
 
@@ -167,6 +167,13 @@ vx_status vx_ctx_destroy(vx_ctx* ctx);
 /* Set (NULL resets) / read the context's trace launch tuning. */
 vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning* tuning);
 vx_status vx_ctx_get_trace_tuning(const vx_ctx* ctx, vx_trace_tuning* tuning);
+/* Kernel timing of the ray-parallel trace (no counterpart in the reference):
+ * while enabled, every vx_trace / vx_batch_launch_direct records HIP events
+ * around its march kernel on the launch stream. vx_ctx_profile(ctx, 1) starts
+ * a new collection, 0 stops it; vx_ctx_profile_read waits for the recorded
+ * launches and returns their total march-kernel milliseconds and count. */
+vx_status vx_ctx_profile(vx_ctx* ctx, int32_t enable);
+vx_status vx_ctx_profile_read(vx_ctx* ctx, double* march_ms, uint32_t* launches);
 
 /* ---------------------------------------------------------------- models
  * VoxAsset (VoxAsset.h:8-62): R8 material volume with 3 mips. Dims are rounded

@@ -1,0 +1,137 @@
+"""Ray-parallel trace (march grid with out-of-bounds border bits, LDS ray ring,
+per-ray result slots, per-pixel resolve) bit for bit against the oracle:
+odd and even volume sizes smaller than the model (rays leave through every
+face), every launch tuning, and after each kind of volume edit (full upload,
+region upload, rebuild with a moved instance, clear)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import IDENTITY, assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+TUNINGS = [dict(kernel=2),
+           dict(kernel=2, block=512, refill=1, steps=2),
+           dict(kernel=2, refill=64, steps=8),
+           dict(kernel=2, refill=7, steps=4, blocks_per_cu=1)]
+
+
+@pytest.fixture(scope="module")
+def api():
+    from voxelengine_amd import api as A
+    return A
+
+
+def _model(api, ctx, dims, seed):
+    m = api.Model(ctx, *dims)
+    m.generate(seed=seed, terrain_fill=0.25, nboxes=30, box_min=2, box_max=12)
+    m.build_mips()
+    return m
+
+
+def _trace_both(api, ctx, oracle, occ, occ_h, model, variant, params, w=72, h=52):
+    from voxelengine_amd import configs, views
+    vd = views.scene_view(model.dims, w, h, frame=variant, variant=variant)
+    g = api.GBuffer(w, h)
+    api.primary(ctx, model, vd, g)
+    pal = configs.make_palette()
+    vis = api.Visibility(w, h, fetches=True)
+    api.trace(ctx, occ, vd, g, torch.from_numpy(pal).cuda(), params, vis)
+    ref = oracle.trace(occ_h, vd, params, g.pos.cpu().numpy(), g.normal.cpu().numpy(), g.material.cpu().numpy(),
+                       pal)
+    got = (vis.shadow.cpu().numpy(), vis.ao.cpu().numpy(), vis.spec.cpu().numpy(),
+           vis.fetches.cpu().numpy().astype(np.uint32))
+    return got, ref
+
+
+def _assert_same(got, ref, what):
+    for name, a, b in zip(("shadow", "ao", "spec", "fetches"), got, ref):
+        assert_bitwise(a, b, f"{what} {name}")
+
+
+@pytest.mark.parametrize("vol_dims", [(45, 37, 43), (48, 40, 44), (17, 50, 9)])
+def test_march_volume_sizes_and_tunings(api, ctx, oracle, vol_dims):
+    from voxelengine_amd import configs
+    model = _model(api, ctx, (96, 80, 88), seed=11)
+    vol = model.download(0)
+    occ = api.Volume(ctx, *vol_dims, nlevels=1)
+    occ.clear()
+    occ.rebuild([api.make_instance(model.mip_ptr(0), model.dims, IDENTITY, IDENTITY)])
+    occ.build_levels()
+    occ_h = np.zeros(vol_dims[::-1], np.uint8)
+    oracle.rebuild(occ_h, [oracle.make_instance(vol, IDENTITY, IDENTITY)])
+    assert_bitwise(occ.download(), occ_h, "occupancy")
+    params = configs.trace_params(n_ao=5, ao_maxt=30.0, march="dda")
+    try:
+        for ti, tu in enumerate(TUNINGS):
+            ctx.set_tuning(**tu)
+            for variant in range(4):
+                got, ref = _trace_both(api, ctx, oracle, occ, occ_h, model, variant, params)
+                _assert_same(got, ref, f"dims {vol_dims} tuning {tu} view {variant}")
+    finally:
+        ctx.reset_tuning()
+
+
+def test_march_after_volume_edits(api, ctx, oracle):
+    from voxelengine_amd import configs
+    model = _model(api, ctx, (64, 64, 64), seed=12)
+    vol = model.download(0)
+    S = (32, 32, 32)
+    params = configs.trace_params(n_ao=4, ao_maxt=24.0, march="dda")
+    occ = api.Volume(ctx, *S, nlevels=2)
+    rng = np.random.default_rng(3)
+
+    # full upload of random texels
+    host = (rng.random(S[::-1]) < 0.04).astype(np.uint8) * rng.integers(1, 256, S[::-1]).astype(np.uint8)
+    occ.upload_region(host, [(0, 0, 0, S[0], S[1], S[2])])
+    got, ref = _trace_both(api, ctx, oracle, occ, host, model, 0, params)
+    _assert_same(got, ref, "full upload")
+
+    # region upload over part of it
+    host2 = host.copy()
+    box = (5, 3, 7, 20, 11, 9)
+    host2[box[2]:box[2] + box[5], box[1]:box[1] + box[4], box[0]:box[0] + box[3]] = 0xFF
+    occ.upload_region(host2, [box])
+    got, ref = _trace_both(api, ctx, oracle, occ, host2, model, 1, params)
+    _assert_same(got, ref, "region upload")
+
+    # rebuild from the model, then move the instance (erase old, write new)
+    occ.clear()
+    occ.rebuild([api.make_instance(model.mip_ptr(0), model.dims, IDENTITY, IDENTITY)])
+    occ_h = np.zeros(S[::-1], np.uint8)
+    oracle.rebuild(occ_h, [oracle.make_instance(vol, IDENTITY, IDENTITY)])
+    moved = np.eye(4, dtype=np.float32)
+    moved[3, :3] = (0.9, -0.6, 1.3)          # column-major translation (world units of 10 voxels)
+    moved = moved.reshape(16)
+    occ.rebuild([api.make_instance(model.mip_ptr(0), model.dims, IDENTITY, moved)])
+    oracle.rebuild(occ_h, [oracle.make_instance(vol, IDENTITY, moved)])
+    assert_bitwise(occ.download(), occ_h, "moved occupancy")
+    got, ref = _trace_both(api, ctx, oracle, occ, occ_h, model, 2, params)
+    _assert_same(got, ref, "moved instance")
+
+    # clear: nothing occludes
+    occ.clear()
+    got, ref = _trace_both(api, ctx, oracle, occ, np.zeros(S[::-1], np.uint8), model, 3, params)
+    _assert_same(got, ref, "cleared")
+    hit = ref[0] > 0
+    assert hit.any() and (ref[1][hit] == 1.0).all()
+
+
+def test_march_profile_counts_launches(api, ctx):
+    from voxelengine_amd import configs, views
+    model = _model(api, ctx, (32, 32, 32), seed=13)
+    occ = api.Volume(ctx, 16, 16, 16, nlevels=0)
+    occ.rebuild([api.make_instance(model.mip_ptr(0), model.dims, IDENTITY, IDENTITY)])
+    vd = views.scene_view(model.dims, 40, 30)
+    g = api.GBuffer(40, 30)
+    api.primary(ctx, model, vd, g)
+    pal = torch.from_numpy(configs.make_palette()).cuda()
+    params = configs.trace_params(n_ao=2, ao_maxt=8.0, march="dda")
+    vis = api.Visibility(40, 30)
+    ctx.profile(True)
+    for _ in range(3):
+        api.trace(ctx, occ, vd, g, pal, params, vis)
+    ms, n = ctx.profile_read()
+    ctx.profile(False)
+    assert n == 3 and ms > 0.0

@@ -75,7 +75,7 @@ class BatchItem(C.Structure):
 
 class TraceTuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
-                ("no_lds_levels", C.c_int32), ("pad", C.c_int32 * 3)]
+                ("steps", C.c_int32), ("pad", C.c_int32 * 3)]
 
 
 # Every symbol the header declares, with its signature.
@@ -86,6 +86,8 @@ SIGNATURES = {
     "vx_ctx_destroy": ([P], C.c_int32),
     "vx_ctx_set_trace_tuning": ([P, C.POINTER(TraceTuning)], C.c_int32),
     "vx_ctx_get_trace_tuning": ([P, C.POINTER(TraceTuning)], C.c_int32),
+    "vx_ctx_profile": ([P, C.c_int32], C.c_int32),
+    "vx_ctx_profile_read": ([P, C.POINTER(C.c_double), C.POINTER(C.c_uint32)], C.c_int32),
     "vx_model_create": ([P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(P)], C.c_int32),
     "vx_model_destroy": ([P], C.c_int32),
     "vx_model_dims": ([P, C.POINTER(C.c_int32)], C.c_int32),

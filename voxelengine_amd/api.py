@@ -41,18 +41,27 @@ class Context:
         self.h = h
         self.device = device
 
-    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, no_lds_levels=0):
-        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, no_lds_levels)
+    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, steps=0):
+        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, steps)
         check(L.lib().vx_ctx_set_trace_tuning(self.h, C.byref(t)))
 
     def reset_tuning(self):
         check(L.lib().vx_ctx_set_trace_tuning(self.h, None))
 
+    def profile(self, enable: bool = True):
+        """Start (or stop) timing the march kernel of every trace launch."""
+        check(L.lib().vx_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        """(total march-kernel ms, launches) since profiling was enabled."""
+        ms, n = C.c_double(), C.c_uint32()
+        check(L.lib().vx_ctx_profile_read(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def tuning(self) -> dict:
         t = L.TraceTuning()
         check(L.lib().vx_ctx_get_trace_tuning(self.h, C.byref(t)))
-        return dict(kernel=t.kernel, block=t.block, blocks_per_cu=t.blocks_per_cu, refill=t.refill,
-                    no_lds_levels=t.no_lds_levels)
+        return dict(kernel=t.kernel, block=t.block, blocks_per_cu=t.blocks_per_cu, refill=t.refill, steps=t.steps)
 
     def close(self):
         if self.h:

@@ -1,8 +1,8 @@
 """Autotuner for the trace launch shape.
 
 Measures candidate vx_trace_tuning settings (kernel variant, block size,
-persistent blocks per CU, lane-refill threshold, LDS staging of the coarse
-levels) on the caller's own workload with HIP events, keeps the fastest, and
+persistent blocks per CU, lane-refill threshold, DDA steps between refill
+checks) on the caller's own workload with HIP events, keeps the fastest, and
 caches the choice per workload key in a JSON file (VX_TUNE_CACHE, default
 ~/.cache/voxelengine_amd/tune.json). Every candidate computes identical bits;
 only the time differs.
@@ -18,13 +18,12 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "voxelengine_amd
 
 CANDIDATES = [
     dict(kernel=1),
-    dict(kernel=2, block=512, refill=64),
-    dict(kernel=2, block=256, refill=64),
-    dict(kernel=2, block=1024, refill=64),
-    dict(kernel=2, block=512, refill=48),
-    dict(kernel=2, block=512, refill=32),
-    dict(kernel=2, block=512, refill=64, blocks_per_cu=2),
-    dict(kernel=2, block=512, refill=64, no_lds_levels=1),
+    dict(kernel=2, block=256, refill=16, steps=4),
+    dict(kernel=2, block=256, refill=8, steps=4),
+    dict(kernel=2, block=256, refill=32, steps=4),
+    dict(kernel=2, block=256, refill=16, steps=2),
+    dict(kernel=2, block=256, refill=16, steps=8),
+    dict(kernel=2, block=512, refill=16, steps=4),
 ]
 
 

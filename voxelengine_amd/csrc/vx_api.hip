@@ -60,14 +60,41 @@ static vx_status fail(vx_status code, const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 // objects
 // ---------------------------------------------------------------------------
+// Scratch of the ray-parallel trace: per-ray results, the list of rays left
+// to the literal path and the queue counters, sized in ray slots.
+struct MarchBuffers {
+    float* res;
+    uint16_t* fres;
+    uint32_t* deferred;
+    uint32_t* counters;      // 9
+    size_t slots;            // capacity of res / deferred
+    size_t fslots;           // capacity of fres
+};
+
+static void march_buffers_free(MarchBuffers& m) {
+    (void)hipFree(m.res);
+    (void)hipFree(m.fres);
+    (void)hipFree(m.deferred);
+    (void)hipFree(m.counters);
+    std::memset(&m, 0, sizeof(m));
+}
+
 struct vx_ctx {
     int32_t device;
     long long* grad_acc;     // 256*6+6 fixed-point gradient accumulator
-    unsigned long long* deferred;   // deferred-pixel list of the DDA trace (capacity entries)
-    uint32_t* ndeferred;     // its counter
-    size_t deferred_cap;
+    MarchBuffers march;
     vx_trace_tuning tuning;
+    bool profile;            // record events around march kernels
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;
 };
+
+static void profile_clear(vx_ctx* c) {
+    for (auto& p : c->prof) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    c->prof.clear();
+}
 
 struct vx_model {
     vx_ctx* ctx;
@@ -85,7 +112,51 @@ struct vx_volume {
     uint64_t* cells;
     uint64_t* lvl[3];
     size_t ncells;
+    // March grid of the ray-parallel trace: the cells in a linear grid with a
+    // border cell per side and out-of-bounds voxel bits set; kept in step with
+    // cells by every write (clear, upload, rebuild).
+    uint64_t* grid;
+    int32_t gx, gy, gz;
+    size_t ngrid;
 };
+
+// Refreshes the march grid over texel box [t0, t1] (inclusive), clamped to the
+// volume, plus the border cells next to it.
+static hipError_t mgrid_update(const vx_volume* v, const int32_t t0[3], const int32_t t1[3], hipStream_t s) {
+    const int32_t gdim[3] = {v->gx, v->gy, v->gz};
+    const int32_t tdim[3] = {v->sx, v->sy, v->sz};
+    int32_t g0[3], n[3];
+    for (int k = 0; k < 3; ++k) {
+        int32_t lo = std::max(t0[k], 0), hi = std::min(t1[k], tdim[k] - 1);
+        if (hi < lo) return hipSuccess;
+        // fine voxels [2lo, 2hi+1] -> grid cells ((f >> 2) + 1), widened by one cell
+        int32_t c0 = std::max(((2 * lo) >> 2) + 1 - 1, 0);
+        int32_t c1 = std::min(((2 * hi + 1) >> 2) + 1 + 1, gdim[k] - 1);
+        g0[k] = c0;
+        n[k] = c1 - c0 + 1;
+    }
+    size_t total = (size_t)n[0] * n[1] * n[2];
+    OccView V;
+    std::memset(&V, 0, sizeof(V));
+    V.cells = v->cells;
+    V.sx = v->sx; V.sy = v->sy; V.sz = v->sz;
+    V.bx = v->bx; V.by = v->by; V.bz = v->bz;
+    hipLaunchKernelGGL(k_mgrid, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s, v->grid, V, v->gx, v->gy,
+                       g0[0], g0[1], g0[2], n[0], n[1], n[2]);
+    return hipGetLastError();
+}
+
+// The whole march grid (every cell, border included).
+static hipError_t mgrid_full(const vx_volume* v, hipStream_t s) {
+    OccView V;
+    std::memset(&V, 0, sizeof(V));
+    V.cells = v->cells;
+    V.sx = v->sx; V.sy = v->sy; V.sz = v->sz;
+    V.bx = v->bx; V.by = v->by; V.bz = v->bz;
+    hipLaunchKernelGGL(k_mgrid, dim3((uint32_t)((v->ngrid + 255) / 256)), dim3(256), 0, s, v->grid, V, v->gx, v->gy,
+                       0, 0, 0, v->gx, v->gy, v->gz);
+    return hipGetLastError();
+}
 
 static inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -146,14 +217,14 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     if (!c) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
     c->grad_acc = nullptr;
-    c->deferred = nullptr;
-    c->ndeferred = nullptr;
-    c->deferred_cap = 0;
+    std::memset(&c->march, 0, sizeof(c->march));
     std::memset(&c->tuning, 0, sizeof(c->tuning));
+    c->profile = false;
     hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
-    if (me == hipSuccess) me = hipMalloc(&c->ndeferred, sizeof(uint32_t));
+    if (me == hipSuccess) me = hipMalloc(&c->march.counters, 9 * sizeof(uint32_t));
     if (me != hipSuccess) {
         (void)hipFree(c->grad_acc);
+        march_buffers_free(c->march);
         delete c;
         return fail(VX_ERR_OUT_OF_MEMORY, "context scratch allocation failed");
     }
@@ -165,8 +236,8 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     if (!ctx) return VX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipFree(ctx->grad_acc);
-    (void)hipFree(ctx->deferred);
-    (void)hipFree(ctx->ndeferred);
+    march_buffers_free(ctx->march);
+    profile_clear(ctx);
     delete ctx;
     return VX_OK;
 }
@@ -178,10 +249,11 @@ extern "C" vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning*
         return VX_OK;
     }
     VX_REQUIRE(t->kernel >= 0 && t->kernel <= 2, "tuning.kernel %d outside 0..2", t->kernel);
-    VX_REQUIRE(t->block == 0 || t->block == 256 || t->block == 512 || t->block == 1024,
-               "tuning.block %d not 256/512/1024", t->block);
+    VX_REQUIRE(t->block == 0 || t->block == 256 || t->block == 512, "tuning.block %d not 256/512", t->block);
     VX_REQUIRE(t->blocks_per_cu >= 0 && t->blocks_per_cu <= 64, "tuning.blocks_per_cu outside 0..64");
     VX_REQUIRE(t->refill >= 0 && t->refill <= 64, "tuning.refill outside 0..64");
+    VX_REQUIRE(t->steps == 0 || t->steps == 2 || t->steps == 4 || t->steps == 8, "tuning.steps %d not 2/4/8",
+               t->steps);
     ctx->tuning = *t;
     return VX_OK;
 }
@@ -189,6 +261,32 @@ extern "C" vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning*
 extern "C" vx_status vx_ctx_get_trace_tuning(const vx_ctx* ctx, vx_trace_tuning* t) {
     VX_REQUIRE(ctx && t, "ctx/tuning is NULL");
     *t = ctx->tuning;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_ctx_profile(vx_ctx* ctx, int32_t enable) {
+    VX_REQUIRE(ctx, "ctx is NULL");
+    if (vx_status st = use_device(ctx)) return st;
+    if (enable) {
+        VX_HIP(hipDeviceSynchronize());
+        profile_clear(ctx);
+    }
+    ctx->profile = enable != 0;
+    return VX_OK;
+}
+
+extern "C" vx_status vx_ctx_profile_read(vx_ctx* ctx, double* march_ms, uint32_t* launches) {
+    VX_REQUIRE(ctx && march_ms && launches, "ctx/march_ms/launches is NULL");
+    if (vx_status st = use_device(ctx)) return st;
+    double total = 0.0;
+    for (auto& p : ctx->prof) {
+        VX_HIP(hipEventSynchronize(p.second));
+        float ms = 0.0f;
+        VX_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+        total += ms;
+    }
+    *march_ms = total;
+    *launches = (uint32_t)ctx->prof.size();
     return VX_OK;
 }
 
@@ -333,6 +431,12 @@ extern "C" vx_status vx_volume_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32
         v->wy[l] = (int32_t)cdiv(v->cy[l], 4);
         v->wz[l] = (int32_t)cdiv(v->cz[l], 4);
     }
+    // March grid: fine voxels -1 .. 2s+2 per axis, cell = (f >> 2) + 1.
+    v->gx = ((2 * sx + 2) >> 2) + 2;
+    v->gy = ((2 * sy + 2) >> 2) + 2;
+    v->gz = ((2 * sz + 2) >> 2) + 2;
+    v->ngrid = (size_t)v->gx * v->gy * v->gz;
+    v->grid = nullptr;
     hipError_t e = hipMalloc(&v->cells, v->ncells * 8);
     if (e == hipSuccess) e = hipMemset(v->cells, 0, v->ncells * 8);
     for (int l = 0; l < nlevels && e == hipSuccess; ++l) {
@@ -340,8 +444,12 @@ extern "C" vx_status vx_volume_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32
         e = hipMalloc(&v->lvl[l], nw * 8);
         if (e == hipSuccess) e = hipMemset(v->lvl[l], 0, nw * 8);
     }
+    if (e == hipSuccess) e = hipMalloc(&v->grid, v->ngrid * 8);
+    if (e == hipSuccess) e = mgrid_full(v, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         (void)hipFree(v->cells);
+        (void)hipFree(v->grid);
         for (int l = 0; l < 3; ++l) (void)hipFree(v->lvl[l]);
         delete v;
         return fail(VX_ERR_OUT_OF_MEMORY, "volume allocation failed: %s", hipGetErrorString(e));
@@ -354,6 +462,7 @@ extern "C" vx_status vx_volume_destroy(vx_volume* v) {
     if (!v) return VX_OK;
     (void)hipSetDevice(v->ctx->device);
     (void)hipFree(v->cells);
+    (void)hipFree(v->grid);
     for (int l = 0; l < 3; ++l) (void)hipFree(v->lvl[l]);
     delete v;
     return VX_OK;
@@ -375,6 +484,7 @@ extern "C" vx_status vx_volume_clear(vx_volume* v, void* stream) {
         hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(nw, 256)), dim3(256), 0, S(stream), v->lvl[l], nw, 0ull);
     }
     VX_LAUNCHED("k_fill_u64");
+    if (hipError_t e = mgrid_full(v, S(stream))) return fail(VX_ERR_DEVICE, "march grid: %s", hipGetErrorString(e));
     return VX_OK;
 }
 
@@ -405,6 +515,11 @@ extern "C" vx_status vx_volume_upload_region(vx_volume* v, const uint8_t* host, 
         hipLaunchKernelGGL(k_bytes_to_cells, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), v->cells, stage, V, r.x,
                            r.y, r.z, (int32_t)r.w, (int32_t)r.h, (int32_t)r.d);
         e = hipGetLastError();
+        if (e == hipSuccess) {
+            const int32_t t0[3] = {r.x, r.y, r.z};
+            const int32_t t1[3] = {r.x + (int32_t)r.w - 1, r.y + (int32_t)r.h - 1, r.z + (int32_t)r.d - 1};
+            e = mgrid_update(v, t0, t1, S(stream));
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
     (void)hipFree(stage);
@@ -572,6 +687,32 @@ extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, ui
             e = hipGetLastError();
         }
     }
+    // March grid over the conservative footprint boxes (fine voxels -> texels):
+    // one refresh per operation, or one over their union for long lists.
+    if (e == hipSuccess) {
+        auto texel_box = [](const PlannedOp& p, int32_t t0[3], int32_t t1[3]) {
+            for (int k = 0; k < 3; ++k) {
+                int64_t lo = std::max<int64_t>(p.lo[k], -8), hi = std::min<int64_t>(p.hi[k], 1ll << 20);
+                t0[k] = (int32_t)(lo >> 1) - 1;
+                t1[k] = (int32_t)(hi >> 1) + 1;
+            }
+        };
+        if (ops.size() <= 64) {
+            for (size_t j = 0; j < ops.size() && e == hipSuccess; ++j) {
+                int32_t t0[3], t1[3];
+                texel_box(ops[j], t0, t1);
+                e = mgrid_update(v, t0, t1, S(stream));
+            }
+        } else {
+            int32_t u0[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, u1[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+            for (const PlannedOp& p : ops) {
+                int32_t t0[3], t1[3];
+                texel_box(p, t0, t1);
+                for (int k = 0; k < 3; ++k) { u0[k] = std::min(u0[k], t0[k]); u1[k] = std::max(u1[k], t1[k]); }
+            }
+            e = mgrid_update(v, u0, u1, S(stream));
+        }
+    }
     if (e == hipSuccess)
         e = hipMemcpyAsync(aabb_h.data(), aabb_d, aabb_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, S(stream));
     if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
@@ -685,31 +826,6 @@ extern "C" vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_vie
     return VX_OK;
 }
 
-// Levels staged in LDS: the finest level whose words, with all coarser ones,
-// fit the budget.
-static const size_t kLdsBudget = 48 * 1024;
-
-static void plan_lds(const vx_volume* v, bool stage, DdaTraceArgs& D) {
-    D.lds_first = v->nlevels;
-    D.lds_words = 0;
-    for (int l = 0; l < 3; ++l) D.lds_off[l] = 0;
-    if (!stage) return;
-    for (int first = 0; first < v->nlevels; ++first) {
-        size_t words = 0;
-        for (int l = first; l < v->nlevels; ++l) words += (size_t)v->wx[l] * v->wy[l] * v->wz[l];
-        if (words * 8 <= kLdsBudget) {
-            D.lds_first = first;
-            uint32_t off = 0;
-            for (int l = first; l < v->nlevels; ++l) {
-                D.lds_off[l] = off;
-                off += (uint32_t)((size_t)v->wx[l] * v->wy[l] * v->wz[l]);
-            }
-            D.lds_words = off;
-            break;
-        }
-    }
-}
-
 static vx_status persistent_grid(vx_ctx* ctx, const void* kernel, int block, size_t lds, int want_per_cu,
                                  uint32_t& grid) {
     int cus = 0, per_cu = 0;
@@ -722,41 +838,100 @@ static vx_status persistent_grid(vx_ctx* ctx, const void* kernel, int block, siz
     return VX_OK;
 }
 
-// Deferred-pixel list sized to the view (grows; the previous list is released
-// after the device finishes with it).
-static vx_status ensure_deferred(vx_ctx* ctx, size_t npx) {
-    if (ctx->deferred_cap >= npx) return VX_OK;
+// Grows the ray-parallel scratch to `slots` ray slots (and fetch counts when
+// asked); buffers in use by queued work are released after a device sync.
+static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
+    const bool grow = m.slots < slots, fgrow = fetches && m.fslots < slots;
+    if (!grow && !fgrow) return VX_OK;
     VX_HIP(hipDeviceSynchronize());
-    (void)hipFree(ctx->deferred);
-    ctx->deferred = nullptr;
-    ctx->deferred_cap = 0;
-    VX_HIP(hipMalloc(&ctx->deferred, npx * sizeof(unsigned long long)));
-    ctx->deferred_cap = npx;
+    if (grow) {
+        (void)hipFree(m.res);
+        (void)hipFree(m.deferred);
+        m.res = nullptr;
+        m.deferred = nullptr;
+        m.slots = 0;
+        VX_HIP(hipMalloc(&m.res, slots * sizeof(float)));
+        VX_HIP(hipMalloc(&m.deferred, slots * sizeof(uint32_t)));
+        m.slots = slots;
+    }
+    if (fgrow) {
+        (void)hipFree(m.fres);
+        m.fres = nullptr;
+        m.fslots = 0;
+        VX_HIP(hipMalloc(&m.fres, slots * sizeof(uint16_t)));
+        m.fslots = slots;
+    }
+    if (!m.counters) VX_HIP(hipMalloc(&m.counters, 9 * sizeof(uint32_t)));
     return VX_OK;
 }
 
-static vx_status launch_trace_dda(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T, hipStream_t stream) {
-    if (vx_status st = ensure_deferred(ctx, (size_t)T.view.width * T.view.height)) return st;
+static MarchItem make_march_item(const vx_volume* vol, const TraceArgs& T, uint32_t tile0) {
+    MarchItem I;
+    std::memset(&I, 0, sizeof(I));
+    I.V = occ_view(vol);
+    I.T = T;
+    I.grid = vol->grid;
+    I.gx = vol->gx;
+    I.gxy = vol->gx * vol->gy;
+    I.glast = (uint32_t)(vol->ngrid - 1);
+    I.lim[0] = 2 * vol->sx + 1; I.lim[1] = 2 * vol->sy + 1; I.lim[2] = 2 * vol->sz + 1;
+    I.tiles_x = cdiv((uint32_t)T.view.width, 8);
+    I.ntiles = I.tiles_x * cdiv((uint32_t)T.view.height, 8);
+    I.tile0 = tile0;
+    return I;
+}
+
+struct MarchShape {
+    int block, steps;
+    uint32_t grid, rest_grid;
+};
+
+static vx_status march_shape(vx_ctx* ctx, bool single, MarchShape& sh) {
     const vx_trace_tuning& tu = ctx->tuning;
-    int block = (tu.block == 256 || tu.block == 1024) ? tu.block : 512;
-    DdaTraceArgs D;
-    D.deferred = ctx->deferred;
-    D.ndeferred = ctx->ndeferred;
-    D.t = T;
-    D.tiles_x = cdiv((uint32_t)T.view.width, 8);
-    D.ntiles = D.tiles_x * cdiv((uint32_t)T.view.height, 8);
-    D.band = cdiv(D.ntiles, 8);
-    D.refill = (tu.refill >= 1 && tu.refill <= 64) ? tu.refill : 64;
-    plan_lds(vol, tu.no_lds_levels == 0, D);
-    size_t lds = (size_t)D.lds_words * 8;
-    uint32_t grid = 0;
-    if (vx_status st = persistent_grid(ctx, trace_dda_kernel(block), block, lds, tu.blocks_per_cu, grid)) return st;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, ctx->ndeferred, 1);
-    hipError_t e = launch_trace_dda_kernel(block, grid, lds, stream, occ_view(vol), D);
-    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of k_trace_dda failed: %s", hipGetErrorString(e));
-    hipLaunchKernelGGL(k_trace_list, dim3(64), dim3(256), 0, stream, occ_view(vol), T, ctx->deferred,
-                       ctx->ndeferred);
-    VX_LAUNCHED("k_trace_list");
+    sh.block = tu.block == 512 ? 512 : 256;
+    sh.steps = tu.steps ? tu.steps : 4;
+    sh.rest_grid = 64;
+    return persistent_grid(ctx, march_kernel(sh.block, sh.steps, single), sh.block, 0, tu.blocks_per_cu, sh.grid);
+}
+
+static void march_common(MarchArgs& A, const vx_trace_params& p, const vx_trace_tuning& tu) {
+    A.nrays = p.n_shadow + p.n_ao + p.n_spec;
+    A.refill = (tu.refill >= 1 && tu.refill <= 64) ? tu.refill : 16;
+    A.maxt[0] = p.shadow_maxt;
+    A.maxt[1] = p.ao_maxt;
+    A.maxt[2] = p.spec_maxt;
+}
+
+static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T, hipStream_t stream) {
+    MarchArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.item0 = make_march_item(vol, T, 0);
+    march_common(A, T.params, ctx->tuning);
+    A.items = nullptr;
+    A.nitems = 1;
+    A.ntiles = A.item0.ntiles;
+    A.band = cdiv(A.ntiles, 8);
+    const size_t slots = (size_t)A.ntiles * (size_t)A.nrays * 64;
+    VX_REQUIRE(slots < (1ull << 32), "view needs %zu ray slots, more than 2^32", slots);
+    if (vx_status st = ensure_march(ctx->march, slots, T.fetches != nullptr)) return st;
+    A.counters = ctx->march.counters;
+    A.deferred = ctx->march.deferred;
+    A.res = ctx->march.res;
+    A.fres = T.fetches ? ctx->march.fres : nullptr;
+    MarchShape sh;
+    if (vx_status st = march_shape(ctx, true, sh)) return st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->profile) {
+        VX_HIP(hipEventCreate(&e0));
+        if (hipError_t ce = hipEventCreate(&e1)) {
+            (void)hipEventDestroy(e0);
+            return fail(VX_ERR_DEVICE, "hipEventCreate failed: %s", hipGetErrorString(ce));
+        }
+        ctx->prof.emplace_back(e0, e1);
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, A.counters, 9);
+    hipError_t e = launch_march(sh.block, sh.steps, true, sh.grid, sh.rest_grid, stream, A, e0, e1);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of the ray-parallel trace failed: %s", hipGetErrorString(e));
     return VX_OK;
 }
 
@@ -790,7 +965,7 @@ extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* 
     const bool all_dda = (params->n_shadow == 0 || params->shadow_march == VX_MARCH_DDA) &&
                          (params->n_ao == 0 || params->ao_march == VX_MARCH_DDA) &&
                          (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA);
-    if (all_dda && ctx->tuning.kernel != 1) return launch_trace_dda(ctx, vol, A, S(stream));
+    if (all_dda && ctx->tuning.kernel != 1) return launch_trace_march(ctx, vol, A, S(stream));
     hipLaunchKernelGGL(k_trace, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
     VX_LAUNCHED("k_trace");
     return VX_OK;
@@ -862,12 +1037,10 @@ extern "C" vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const v
 struct vx_batch {
     vx_ctx* ctx;
     uint32_t n;
-    BatchDesc* descs;            // device
-    uint32_t* counters;          // device: 8 band counters + deferred count
-    unsigned long long* deferred;
-    BatchArgs args;
-    int block;
-    uint32_t grid;
+    MarchItem* items;            // device
+    MarchBuffers march;
+    MarchArgs args;
+    MarchShape shape;
     hipStream_t cap;             // capture stream
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -879,20 +1052,19 @@ static void batch_free(vx_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap) (void)hipStreamDestroy(b->cap);
-    (void)hipFree(b->descs);
-    (void)hipFree(b->counters);
-    (void)hipFree(b->deferred);
+    (void)hipFree(b->items);
+    march_buffers_free(b->march);
     delete b;
 }
 
-// The kernels of one batched launch: clear the queue counters, trace, then the
-// deferred pixels.
-static hipError_t batch_enqueue(vx_batch* b, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, b->counters, 9);
-    hipError_t e = launch_trace_batch_kernel(b->block, b->grid, s, b->args);
+// The kernels of one batched launch: clear the queue counters (a fill kernel,
+// no memset node), march, listed rays, resolve.
+static hipError_t batch_enqueue(vx_batch* b, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, b->march.counters, 9);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_trace_list_batch, dim3(64), dim3(256), 0, s, b->descs, b->deferred, b->counters + 8);
-    return hipGetLastError();
+    return launch_march(b->shape.block, b->shape.steps, false, b->shape.grid, b->shape.rest_grid, s, b->args, e0,
+                        e1);
 }
 
 extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
@@ -905,56 +1077,60 @@ extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, ui
                    (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA),
                "the batched trace runs the DDA marcher for every ray class");
     *out = nullptr;
-    std::vector<BatchDesc> h(n);
-    uint64_t tiles = 0, pixels = 0;
+    std::vector<MarchItem> h(n);
+    uint64_t tiles = 0;
+    bool fetches = false;
     for (uint32_t i = 0; i < n; ++i) {
         const vx_batch_item& it = items[i];
         VX_REQUIRE(it.volume, "item %u: volume is NULL", i);
         if (vx_status st = check_view(&it.view)) return st;
         VX_REQUIRE(it.gbuf.pos && it.gbuf.normal && it.gbuf.material && it.out.shadow && it.out.ao && it.out.spec,
                    "item %u: NULL buffer", i);
-        BatchDesc& D = h[i];
-        std::memset(&D, 0, sizeof(D));
-        D.V = occ_view(it.volume);
-        D.T.view = it.view;
-        D.T.params = *params;
-        D.T.tiles = tile_map(it.view.width, it.view.height);
-        D.T.pos = it.gbuf.pos; D.T.nrm = it.gbuf.normal; D.T.mat = it.gbuf.material;
-        D.T.palette = palette;
-        D.T.shadow = it.out.shadow; D.T.ao = it.out.ao; D.T.spec = it.out.spec; D.T.fetches = it.out.fetches;
-        D.tiles_x = cdiv((uint32_t)it.view.width, 8);
-        D.ntiles = D.tiles_x * cdiv((uint32_t)it.view.height, 8);
-        D.tile0 = (uint32_t)tiles;
-        tiles += D.ntiles;
-        pixels += (uint64_t)it.view.width * it.view.height;
+        TraceArgs T;
+        std::memset(&T, 0, sizeof(T));
+        T.view = it.view;
+        T.params = *params;
+        T.tiles = tile_map(it.view.width, it.view.height);
+        T.pos = it.gbuf.pos; T.nrm = it.gbuf.normal; T.mat = it.gbuf.material;
+        T.palette = palette;
+        T.shadow = it.out.shadow; T.ao = it.out.ao; T.spec = it.out.spec; T.fetches = it.out.fetches;
+        fetches = fetches || it.out.fetches;
+        h[i] = make_march_item(it.volume, T, (uint32_t)tiles);
+        tiles += h[i].ntiles;
         VX_REQUIRE(tiles < (1ull << 31), "batch too large");
     }
+    const int32_t nrays = params->n_shadow + params->n_ao + params->n_spec;
+    const size_t slots = (size_t)tiles * (size_t)nrays * 64;
+    VX_REQUIRE(slots < (1ull << 32), "batch needs %zu ray slots, more than 2^32", slots);
     if (vx_status st = use_device(ctx)) return st;
     vx_batch* b = new (std::nothrow) vx_batch();
     if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
     std::memset(b, 0, sizeof(*b));
     b->ctx = ctx;
     b->n = n;
-    hipError_t e = hipMalloc(&b->descs, n * sizeof(BatchDesc));
-    if (e == hipSuccess) e = hipMalloc(&b->counters, 9 * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&b->deferred, pixels * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemcpy(b->descs, h.data(), n * sizeof(BatchDesc), hipMemcpyHostToDevice);
+    hipError_t e = hipMalloc(&b->items, n * sizeof(MarchItem));
+    if (e == hipSuccess) e = hipMemcpy(b->items, h.data(), n * sizeof(MarchItem), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         batch_free(b);
         return fail(VX_ERR_OUT_OF_MEMORY, "batch allocation failed: %s", hipGetErrorString(e));
     }
-    const vx_trace_tuning& tu = ctx->tuning;
-    b->block = (tu.block == 256 || tu.block == 1024) ? tu.block : 512;
-    BatchArgs& B = b->args;
-    B.descs = b->descs;
-    B.nitems = n;
-    B.ntiles = (uint32_t)tiles;
-    B.band = cdiv(tiles, 8);
-    B.counters = b->counters;
-    B.deferred = b->deferred;
-    B.ndeferred = b->counters + 8;
-    B.refill = (tu.refill >= 1 && tu.refill <= 64) ? tu.refill : 64;
-    if (vx_status st = persistent_grid(ctx, trace_batch_kernel(b->block), b->block, 0, tu.blocks_per_cu, b->grid)) {
+    if (vx_status st = ensure_march(b->march, slots, fetches)) {
+        batch_free(b);
+        return st;
+    }
+    MarchArgs& A = b->args;
+    std::memset(&A, 0, sizeof(A));
+    A.item0 = h[0];
+    march_common(A, *params, ctx->tuning);
+    A.items = b->items;
+    A.nitems = n;
+    A.ntiles = (uint32_t)tiles;
+    A.band = cdiv(tiles, 8);
+    A.counters = b->march.counters;
+    A.deferred = b->march.deferred;
+    A.res = b->march.res;
+    A.fres = fetches ? b->march.fres : nullptr;
+    if (vx_status st = march_shape(ctx, false, b->shape)) {
         batch_free(b);
         return st;
     }
@@ -986,7 +1162,17 @@ extern "C" vx_status vx_batch_launch(vx_batch* b, void* stream) {
 extern "C" vx_status vx_batch_launch_direct(vx_batch* b, void* stream) {
     VX_REQUIRE(b, "batch is NULL");
     if (vx_status st = use_device(b->ctx)) return st;
-    hipError_t e = batch_enqueue(b, S(stream));
+    vx_ctx* ctx = b->ctx;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->profile) {
+        VX_HIP(hipEventCreate(&e0));
+        if (hipError_t ce = hipEventCreate(&e1)) {
+            (void)hipEventDestroy(e0);
+            return fail(VX_ERR_DEVICE, "hipEventCreate failed: %s", hipGetErrorString(ce));
+        }
+        ctx->prof.emplace_back(e0, e1);
+    }
+    hipError_t e = batch_enqueue(b, S(stream), e0, e1);
     if (e != hipSuccess) return fail(VX_ERR_DEVICE, "batch launch failed: %s", hipGetErrorString(e));
     return VX_OK;
 }

@@ -36,37 +36,30 @@ struct TraceArgs {
     uint32_t* fetches;
 };
 
-// Persistent DDA trace: 8x8-pixel tiles, split into 8 contiguous bands (one
-// per XCD); the waves of an XCD stream the tiles of its band.
-struct DdaTraceArgs {
-    TraceArgs t;
-    uint32_t tiles_x;        // 8x8 tiles per row
-    uint32_t ntiles;
-    uint32_t band;           // tiles per XCD band
-    int32_t lds_first;       // finest level staged in LDS (0-based), nlevels if none
-    uint32_t lds_off[3];     // word offset of each staged level in LDS
-    uint32_t lds_words;
-    int32_t refill;          // refill idle lanes once this many are idle
-    unsigned long long* deferred;   // pixels with a ray outside the fast path (item << 32 | pixel)
-    uint32_t* ndeferred;
+// One item of the ray-parallel trace: a volume and a view.
+struct MarchItem {
+    OccView V;                 // occupancy (literal path of listed rays)
+    TraceArgs T;               // view, params, G-buffer, outputs
+    const uint64_t* grid;      // march grid (border cells included)
+    int32_t gx, gxy;           // march grid strides in words
+    uint32_t glast;            // last march grid word
+    int32_t lim[3];            // 2*s+1: last in-bounds fine voxel per axis
+    uint32_t tiles_x, ntiles;  // 8x8 tiles of the view
+    uint32_t tile0;            // first global tile
 };
 
-// One item of a batched trace: a volume and a view; tile0 = first global tile.
-struct BatchDesc {
-    OccView V;
-    TraceArgs T;
-    uint32_t tiles_x, ntiles, tile0, pad;
-};
-
-struct BatchArgs {
-    const BatchDesc* descs;
-    uint32_t nitems;
-    uint32_t ntiles;
+struct MarchArgs {
+    MarchItem item0;           // the item of a single-view launch (a copy of items[0] otherwise)
+    const MarchItem* items;    // device array of a batched launch
+    uint32_t nitems, ntiles;   // items, global tiles
     uint32_t band;             // tiles per XCD band
-    uint32_t* counters;        // 8 band counters
-    unsigned long long* deferred;
-    uint32_t* ndeferred;
-    int32_t refill;
+    uint32_t* counters;        // [0..7] band counters, [8] listed-ray count
+    uint32_t* deferred;        // slots of rays left to the literal path
+    float* res;                // per-ray value, slot = ((tile * nrays + r) << 6) | pixel-in-tile
+    uint16_t* fres;            // per-ray fetch count, or null
+    int32_t nrays;
+    int32_t refill;            // not-running lanes that trigger a refill
+    float maxt[3];             // shadow, AO, spec
 };
 
 struct PrimaryArgs {
@@ -112,10 +105,6 @@ __global__ void k_level_up(uint64_t* words, const uint64_t* lower, int32_t lwx, 
                            int32_t nwz);
 __global__ void k_trace_rays(OccView V, const float* rays, uint32_t n, int32_t march, vx_ray_result* out);
 __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A);
-__global__ void __launch_bounds__(256) k_trace_list(OccView V, TraceArgs A, const unsigned long long* list,
-                                                    const uint32_t* count);
-__global__ void __launch_bounds__(256) k_trace_list_batch(const BatchDesc* descs, const unsigned long long* list,
-                                                          const uint32_t* count);
 __global__ void k_zero_u32(uint32_t* p, int n);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
 __global__ void k_shade_fwd(ShadeArgs A);
@@ -123,11 +112,14 @@ __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);
 __global__ void k_shade_bwd_finish(const long long* acc, float* dpal, float* dlight);
 __global__ void k_zero_i64(long long* p, int n);
 
-// Persistent DDA trace kernel instantiated for 256/512/1024-thread blocks.
-const void* trace_dda_kernel(int block);
-hipError_t launch_trace_dda_kernel(int block, uint32_t grid, size_t lds, hipStream_t stream, const OccView& V,
-                                   const DdaTraceArgs& A);
-const void* trace_batch_kernel(int block);
-hipError_t launch_trace_batch_kernel(int block, uint32_t grid, hipStream_t stream, const BatchArgs& B);
+__global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32_t g0x, int32_t g0y, int32_t g0z,
+                        int32_t nx, int32_t ny, int32_t nz);
+
+// Ray-parallel trace, instantiated for 256/512-thread blocks and 2/4/8 DDA
+// steps between refills; launch_march enqueues march, listed rays and resolve
+// (e0 / e1, when given, are recorded around the march kernel).
+const void* march_kernel(int block, int steps, bool single);
+hipError_t launch_march(int block, int steps, bool single, uint32_t grid, uint32_t rest_grid, hipStream_t s,
+                        const MarchArgs& A, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 }  // namespace vx

@@ -452,14 +452,6 @@ __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A) {
     trace_pixel_exact(V, A, (size_t)py * A.view.width + px);
 }
 
-// Pixels deferred by k_trace_dda (a ray outside its fast path), traced whole.
-__global__ void __launch_bounds__(256) k_trace_list(OccView V, TraceArgs A, const unsigned long long* list,
-                                                    const uint32_t* count) {
-    uint32_t n = *count;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-        trace_pixel_exact(V, A, (uint32_t)list[j]);
-}
-
 VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
     const vx_trace_params& P = A.params;
     uint8_t m = A.mat[i];
@@ -558,54 +550,37 @@ VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
 }
 
 // ===========================================================================
-// Persistent DDA trace: every ray class through raycastShadowVolume semantics
-// (Light.frag:29-81), one ray per lane.
+// Ray-parallel DDA trace: every ray class through raycastShadowVolume
+// semantics (Light.frag:29-81).
 //
-// Fast path. For a ray with all |dir| >= 2^-100 (finite), |origin| < 2^22 and
-// no negative-direction axis starting exactly on a voxel boundary, every
-// t_max is positive and finite for the whole march. Then, exactly:
-//   select_k = (t_max_k == min3(t_max))          (step(): no NaNs)
-//   best_t   = dot(t_max, select) = t, t+t or (t+t)+t for 1, 2 or 3 ties
+// Work is split by ray. A persistent wave takes 8x8-pixel tiles from its XCD's
+// band and generates their rays 64 at a time into a ring in LDS: one ray index
+// over the tile's valid pixels per batch, so the set-up code runs with one ray
+// class and all lanes. A lane whose ray ended takes the next ray from the
+// ring. Every ray owns one result slot (its occlusion value and fetch count),
+// slot = ((tile * nrays + r) << 6) | pixel-in-tile; k_resolve folds a pixel's
+// slots in ray order, so the float sums keep the reference order.
+//
+// Exact fast path. For a ray with every |dir| in [2^-100, 3e38] and |origin| <
+// 2^22, t_max and t_delta stay finite and t_max >= 0 for the whole march, so:
+//   select_k = (t_max_k == min3(t_max))            (step(): no NaNs)
+//   best_t   = dot(t_max, select) = t, t+t or (t+t)+t for 1, 2 or 3 selected,
+//              plus +0 (an unselected t_max*0) unless all 3 are selected
 //   t_max_k += t_delta_k * select_k  ==  select_k ? t_max_k + t_delta_k : t_max_k
-//   current_voxel_k += ivec3(select_k * sign_k) == select_k ? sign_k : 0
-// Pixels with any other ray are traced whole by k_trace_list (literal path).
+//   voxel_k += ivec3(select_k * sign_k) == select_k ? sign_k : 0
+// Other rays are listed and traced afterwards by the literal path (k_march_rest).
+//
+// Voxels come from the march grid: the occupancy cells (u64 = 4x4x4 fine
+// voxels, bit (x&3)|(y&3)<<2|(z&3)<<4) in a linear X-fastest grid with one
+// border cell per side, where every fine voxel outside [0, 2s+1]^3 has its bit
+// set. A ray that starts inside the bounds stops at its first set bit, either
+// an occupied voxel (hit) or the first out-of-bounds voxel (term 2), told
+// apart once at its end, and every voxel it visits lies inside the grid.
+// Lane coordinates carry a +4 bias, so grid cell = biased coordinate >> 2.
 // ===========================================================================
 
-// Per-launch constants of the tile tracer: staged levels, the deferred-pixel
-// list (entries item << 32 | pixel) and the refill threshold.
-struct TileCtx {
-    const uint64_t* s_lvl;
-    int32_t lds_first;
-    uint32_t lds_off[3];
-    unsigned long long* deferred;
-    uint32_t* ndeferred;
-    uint32_t item;
-    int32_t refill;
-};
-
-// A prepared ray: DDA state at the origin (t_max, t_delta, voxel), step signs
-// packed in bits 0..2 (1 = negative), kind in bits 4..5.
-struct PendRay {
-    float tm0, tm1, tm2, td0, td1, td2;
-    int32_t cx, cy, cz;
-    uint32_t meta;
-    float maxt;
-};
-
-// The marching ray. ckey is the packed 4^3-cell coordinate whose occupancy
-// word is held in cw (0 for cells of an empty brick); bkey the packed brick
-// coordinate of the last coarse-level lookup, bempty its result.
-struct LaneRay {
-    float tm0, tm1, tm2, td0, td1, td2;
-    int32_t cx, cy, cz;
-    int32_t stx, sty, stz;
-    int32_t n;
-    float maxt;
-    int32_t bkey;
-    int32_t ckey;
-    bool bempty;
-    uint64_t cw;
-};
+constexpr int kRing = 128;       // ring entries per wave
+constexpr int kRingFields = 10;  // t_max[3], t_delta[3], x|y<<16, z|signs<<16|kind<<19, slot, item
 
 VXD bool fast_ray_ok(f3 o, f3 d) {
     const float lo = 7.8886091e-31f;   // 2^-100
@@ -615,145 +590,27 @@ VXD bool fast_ray_ok(f3 o, f3 d) {
     for (int k = 0; k < 3; ++k) {
         float a = fabsf(dd[k]);
         ok = ok && (a >= lo) && (a <= 3.0e38f) && (fabsf(oo[k]) < 4194304.0f);
-        ok = ok && !(dd[k] < 0.0f && floorf(oo[k]) == oo[k]);
     }
     return ok;
 }
 
-// DDA state at the origin (Light.frag:32-46); |o| < 2^22 so the conversions
-// are exact without saturation.
-VXD void prepare_ray(PendRay& P, f3 o, f3 d, float maxt, int32_t kind) {
-    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, tm[3], td[3];
-    int32_t cv[3];
-    uint32_t meta = (uint32_t)kind << 4;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        bool neg = !(dd[k] > 0.0f);
-        float s = neg ? -1.0f : 1.0f;
-        meta |= neg ? (1u << k) : 0u;
-        td[k] = 1.0f / (dd[k] * s);
-        float fl = floorf(oo[k]);
-        cv[k] = (int32_t)fl;
-        float nb = fl + (neg ? 0.0f : 1.0f);      // vec3(cv) + (sign*0.5 + 0.5)
-        tm[k] = (nb - oo[k]) / dd[k];
-    }
-    P.tm0 = tm[0]; P.tm1 = tm[1]; P.tm2 = tm[2];
-    P.td0 = td[0]; P.td1 = td[1]; P.td2 = td[2];
-    P.cx = cv[0]; P.cy = cv[1]; P.cz = cv[2];
-    P.meta = meta;
-    P.maxt = maxt;
-}
-
-VXD int32_t begin_ray(LaneRay& R, const PendRay& P) {
-    R.tm0 = P.tm0; R.tm1 = P.tm1; R.tm2 = P.tm2;
-    R.td0 = P.td0; R.td1 = P.td1; R.td2 = P.td2;
-    R.cx = P.cx; R.cy = P.cy; R.cz = P.cz;
-    R.stx = (P.meta & 1u) ? -1 : 1;
-    R.sty = (P.meta & 2u) ? -1 : 1;
-    R.stz = (P.meta & 4u) ? -1 : 1;
-    R.n = 0;
-    R.maxt = P.maxt;
-    R.bkey = -1;   // brick keys are < 2^31
-    R.ckey = ((P.cx >> 2) | ((P.cy >> 2) << 11) | ((P.cz >> 2) << 22)) ^ 1;   // differs from the first cell
-    R.bempty = true;
-    R.cw = 0;
-    return (int32_t)((P.meta >> 4) & 3u);
-}
-
-VXD bool brick_empty_staged(const OccView& V, const TileCtx& A, int32_t bx, int32_t by, int32_t bz) {
-    const uint64_t* s_lvl = A.s_lvl;
-    if (V.nlevels == 0) return false;
-    if (A.lds_first == 0) {
-        uint32_t wi = __umul24(__umul24((uint32_t)(bz >> 2), (uint32_t)V.wy[0]) + (uint32_t)(by >> 2), (uint32_t)V.wx[0]) +
-                      (uint32_t)(bx >> 2);
-        uint32_t b = (uint32_t)((bx & 3) | ((by & 3) << 2) | ((bz & 3) << 4));
-        return !((s_lvl[A.lds_off[0] + wi] >> b) & 1ull);
-    }
-    for (int32_t l = V.nlevels - 1; l >= 0; --l) {
-        int32_t sh = 3 * l;
-        int32_t cx = bx >> sh, cy = by >> sh, cz = bz >> sh;
-        uint32_t wi = __umul24(__umul24((uint32_t)(cz >> 2), (uint32_t)V.wy[l]) + (uint32_t)(cy >> 2), (uint32_t)V.wx[l]) +
-                      (uint32_t)(cx >> 2);
-        uint32_t b = (uint32_t)((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
-        uint64_t w = (l >= A.lds_first) ? s_lvl[A.lds_off[l] + wi] : V.lvl[l][wi];
-        if (!((w >> b) & 1ull)) return true;
-    }
-    return false;
-}
-
-// One DDA iteration. Returns 0 to continue, 1 hit, 2 out of bounds, 3 loop end
-// (n == 256 or totalt >= maxt: the x4 restart repeats this pass).
-VXD int32_t dda_step(const OccView& V, const TileCtx& A, LaneRay& R, float& best_t) {
-    float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
-    bool s0 = R.tm0 == tmin, s1 = R.tm1 == tmin, s2 = R.tm2 == tmin;
-    if ((uint32_t)R.cx > (uint32_t)(2 * V.sx + 1) || (uint32_t)R.cy > (uint32_t)(2 * V.sy + 1) ||
-        (uint32_t)R.cz > (uint32_t)(2 * V.sz + 1))
-        return 2;
-    // cell coordinates <= (2*4095+1)>>2 = 2047 fit 11 bits each
-    int32_t ckey = (R.cx >> 2) | ((R.cy >> 2) << 11) | ((R.cz >> 2) << 22);
-    if (ckey != R.ckey) {
-        R.ckey = ckey;
-        int32_t bx = R.cx >> 3, by = R.cy >> 3, bz = R.cz >> 3;
-        int32_t bkey = bx | (by << 10) | (bz << 20);
-        if (bkey != R.bkey) {
-            R.bkey = bkey;
-            R.bempty = brick_empty_staged(V, A, bx, by, bz);
-        }
-        uint64_t w = 0;
-        if (!R.bempty) {
-            uint32_t bi = __umul24(__umul24((uint32_t)bz, (uint32_t)V.by) + (uint32_t)by, (uint32_t)V.bx) + (uint32_t)bx;
-            w = V.cells[(size_t)bi * 8 + sub3(R.cx >> 2, R.cy >> 2, R.cz >> 2)];
-        }
-        R.cw = w;
-    }
-    bool vox = (R.cw >> cell_bit(R.cx, R.cy, R.cz)) & 1ull;
-    best_t = tmin;
-    bool tie = (s0 && s1) || (s0 && s2) || (s1 && s2);
-    if (tie) best_t = (s0 && s1 && s2) ? ((tmin + tmin) + tmin) : (tmin + tmin);
-    if (vox) return 1;
-    R.cx = s0 ? R.cx + R.stx : R.cx;
-    R.cy = s1 ? R.cy + R.sty : R.cy;
-    R.cz = s2 ? R.cz + R.stz : R.cz;
-    R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
-    R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
-    R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
-    R.n++;
-    if (!(R.n < 256 && best_t < R.maxt)) return 3;
-    return 0;
-}
-
-// Per-lane pixel state; the G-buffer is re-read where a ray set-up needs it.
+// Per-lane pixel state of the generation tile.
 struct PixState {
     f3 org;
     uint32_t pk;
     uint32_t i;
-    int32_t r;
-    uint32_t nf;
-    float aosum, spv;
-    uint32_t lit;
 };
 
 enum { RAY_SHADOW = 0, RAY_AO = 1, RAY_SPEC = 2 };
 
-// Pixel set-up shared by its rays (LightAmbient.frag:136-158). Returns false
-// for sky pixels (outputs written). wd receives the shadow direction.
-VXD bool pixel_init(const TraceArgs& A, uint32_t i, PixState& S, f3& wd) {
+// Pixel set-up shared by its rays (LightAmbient.frag:136-158) for a non-sky
+// pixel i; wd receives the shadow direction.
+VXD void pixel_setup(const TraceArgs& A, uint32_t i, PixState& S, f3& wd) {
     const vx_trace_params& P = A.params;
-    uint32_t m = A.mat[i];
-    if (m == 0) {
-        A.shadow[i] = 0; A.ao[i] = 0.0f; A.spec[i] = 0.0f;
-        if (A.fetches) A.fetches[i] = 0;
-        return false;
-    }
     f3 p = load3(A.pos + 3 * (size_t)i);
     f3 n = load3(A.nrm + 3 * (size_t)i);
     S.i = i;
     S.pk = pixkey(P.seed, P.frame, i);
-    S.r = 0;
-    S.nf = 0;
-    S.aosum = 0.0f;
-    S.spv = 1.0f;
-    S.lit = 1;
     float r0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) r0[k] = rand01(S.pk, (uint32_t)k);
@@ -777,7 +634,6 @@ VXD bool pixel_init(const TraceArgs& A, uint32_t i, PixState& S, f3& wd) {
         wcp = mk3(wcp.x + rv.x * 2.5f, wcp.y + rv.y * 2.5f, wcp.z + rv.z * 2.5f);
     }
     S.org = mk3(wcp.x + n.x * bias, wcp.y + n.y * bias, wcp.z + n.z * bias);
-    return true;
 }
 
 // Origin, direction and maxt of ray r >= n_shadow of the pixel (AO or spec).
@@ -821,226 +677,418 @@ VXD int32_t ray_setup(const TraceArgs& A, const PixState& S, int32_t r, f3& o, f
     return RAY_SPEC;
 }
 
-VXD void ray_finish(PixState& S, int32_t kind, int32_t hit, int32_t term, int32_t nt, float t, float maxt) {
-    S.nf += (uint32_t)(term == 0 ? 4 * nt : nt + (term == 1 ? 1 : 0));
-    if (kind == RAY_SHADOW) {
-        S.lit = hit ? 0u : 1u;
-        return;
+// Ray r of pixel S (wd: the pixel's shadow direction).
+VXD int32_t pixel_ray(const TraceArgs& A, const PixState& S, f3 wd, int32_t r, f3& o, f3& d, float& maxt) {
+    if (r < A.params.n_shadow) {
+        o = S.org; d = wd; maxt = A.params.shadow_maxt;
+        return RAY_SHADOW;
     }
+    return ray_setup(A, S, r, o, d, maxt);
+}
+
+// Value a pixel folds for one ray (shadow: lit, AO and spec: min(dist, maxt) /
+// maxt with dist = maxt unless hit) and the ray's fetch count.
+VXD float ray_value(int32_t kind, int32_t term, float t, float maxt) {
+    bool hit = term == 1;
+    if (kind == RAY_SHADOW) return hit ? 0.0f : 1.0f;
     float dist = hit ? gmin(t, maxt) : maxt;
-    float q = gmin(dist, maxt) / maxt;
-    if (kind == RAY_AO) S.aosum = S.aosum + q * q;
-    else S.spv = q;
+    return gmin(dist, maxt) / maxt;
+}
+VXD uint32_t ray_fetches(int32_t term, int32_t nt) {
+    return (uint32_t)(term == 0 ? 4 * nt : nt + (term == 1 ? 1 : 0));
 }
 
-VXD void pixel_write(const TraceArgs& A, const PixState& S) {
-    const vx_trace_params& P = A.params;
-    A.shadow[S.i] = (uint8_t)S.lit;
-    A.ao[S.i] = P.n_ao > 0 ? S.aosum / (float)P.n_ao : 1.0f;
-    A.spec[S.i] = P.n_spec > 0 ? S.spv : 1.0f;
-    if (A.fetches) A.fetches[S.i] = S.nf;
+// best_t of the current voxel from t_max (see the fast-path identities).
+VXD float best_t_of(float tm0, float tm1, float tm2) {
+    float t = fminf(fminf(tm0, tm1), tm2);
+    int32_t ns = (tm0 == t ? 1 : 0) + (tm1 == t ? 1 : 0) + (tm2 == t ? 1 : 0);
+    if (ns == 3) return (t + t) + t;
+    return (ns == 2 ? t + t : t) + 0.0f;
 }
 
-// Prepares the pixel's next ray into P; returns false when that ray is outside
-// the fast path (the pixel is then deferred to the literal path).
-VXD bool prepare_next(const TraceArgs& T, const TileCtx& C, PixState& S, PendRay& P, f3 wd) {
-    f3 o, d;
+template <bool SINGLE>
+VXD const MarchItem& march_item(const MarchArgs& A, uint32_t k) {
+    if (SINGLE) return A.item0;
+    return A.items[k];
+}
+
+// Item of global tile gt (items ordered by tile0).
+template <bool SINGLE>
+VXD uint32_t item_of_tile(const MarchArgs& A, uint32_t gt) {
+    if (SINGLE) return 0;
+    uint32_t lo = 0, hi = A.nitems - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1u) >> 1;
+        if (A.items[mid].tile0 <= gt) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// Pixel index of lane l of tile lt, or false outside the view.
+VXD bool tile_lane_pixel(const MarchItem& I, uint32_t lt, uint32_t l, uint32_t& i) {
+    uint32_t px = (lt % I.tiles_x) * 8u + (l & 7u), py = (lt / I.tiles_x) * 8u + (l >> 3);
+    i = py * (uint32_t)I.T.view.width + px;
+    return px < (uint32_t)I.T.view.width && py < (uint32_t)I.T.view.height;
+}
+
+// Origin voxel and DDA state of ray (o, d) at the origin, Light.frag:32-46
+// (|o| < 2^22, so the int conversion is exact).
+VXD void ray_origin_voxel(f3 o, int32_t cv[3]) {
+    cv[0] = (int32_t)floorf(o.x);
+    cv[1] = (int32_t)floorf(o.y);
+    cv[2] = (int32_t)floorf(o.z);
+}
+
+// Writes the ring entry of a fast-path ray that starts inside the bounds.
+VXD void ring_put_ray(uint32_t (*ring)[kRing], uint32_t e, f3 o, f3 d, const int32_t cv[3], int32_t kind,
+                      uint32_t slot, uint32_t item) {
+    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    uint32_t signs = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        bool neg = !(dd[k] > 0.0f);
+        float s = neg ? -1.0f : 1.0f;
+        signs |= neg ? (1u << k) : 0u;
+        float nb = (float)cv[k] + (neg ? 0.0f : 1.0f);   // vec3(cv) + (sign*0.5 + 0.5)
+        ring[k][e] = __float_as_uint((nb - oo[k]) / dd[k]);
+        ring[3 + k][e] = __float_as_uint(1.0f / (dd[k] * s));
+    }
+    ring[6][e] = (uint32_t)(cv[0] + 4) | ((uint32_t)(cv[1] + 4) << 16);
+    ring[7][e] = (uint32_t)(cv[2] + 4) | (signs << 16) | ((uint32_t)kind << 19);
+    ring[8][e] = slot;
+    ring[9][e] = item;
+}
+
+// The ray a lane marches: coordinates biased by +4; meta = n << 20 | kind << 16
+// | item, so n++ is meta += 1 << 20 and n < 256 is meta < 256 << 20.
+struct MarchRay {
+    float tm0, tm1, tm2, td0, td1, td2;
+    int32_t x, y, z, sx, sy, sz;
+    uint32_t meta;
     float maxt;
-    int32_t kind;
-    if (S.r < T.params.n_shadow) {
-        o = S.org; d = wd; maxt = T.params.shadow_maxt;
-        kind = RAY_SHADOW;
-    } else {
-        kind = ray_setup(T, S, S.r, o, d, maxt);
-    }
-    S.r++;
-    if (fast_ray_ok(o, d)) {
-        prepare_ray(P, o, d, maxt, kind);
-        return true;
-    }
-    uint32_t slot = atomicAdd(C.ndeferred, 1u);
-    C.deferred[slot] = ((unsigned long long)C.item << 32) | (unsigned long long)S.i;
-    return false;
+    uint32_t slot;
+    uint64_t cw;
+};
+
+// Grid word index of biased voxel (x, y, z), clamped into the grid.
+VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gxy, uint32_t glast) {
+    uint32_t ci = __umul24((uint32_t)(z >> 2), gxy) + __umul24((uint32_t)(y >> 2), gx) + (uint32_t)(x >> 2);
+    return min(ci, glast);
 }
 
-// One 8x8-pixel tile per wave; lane l owns pixel l and runs its rays in order.
-// A lane whose ray ends starts its next ray once `refill` lanes wait for one
-// (or no lane is marching); refill 64 keeps the rays of a tile in lockstep.
-VXD void trace_tile(const OccView& V, const TraceArgs& T, const TileCtx& C, uint32_t px, uint32_t py) {
-    const int32_t nrays = T.params.n_shadow + T.params.n_ao + T.params.n_spec;
-    PixState S;
-    LaneRay R;
-    int32_t kind = RAY_SHADOW;
-    bool valid = false, active = false;
-    if (px < (uint32_t)T.view.width && py < (uint32_t)T.view.height) {
-        f3 wd;
-        valid = pixel_init(T, py * (uint32_t)T.view.width + px, S, wd);
-        if (valid && nrays > 0) {
-            PendRay P;
-            valid = prepare_next(T, C, S, P, wd);
-            if (valid) {
-                kind = begin_ray(R, P);
-                active = true;
-            }
-        } else if (valid) {
-            pixel_write(T, S);
-            valid = false;
-        }
-    }
+template <int BLOCK, int STEPS, bool SINGLE>
+__global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
+    constexpr int WAVES = BLOCK / 64;
+    __shared__ uint32_t ring_s[WAVES][kRingFields][kRing];
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t(*ring)[kRing] = ring_s[threadIdx.x >> 6];
+    const int32_t nrays = A.nrays;
+    const uint64_t lane_lt = (1ull << lane) - 1ull;
+
+    // generation cursor (wave-uniform)
+    uint32_t q = blockIdx.x & 7u, misses = 0;
+    bool gen_done = false;
+    uint32_t gitem = 0, gtile = 0;
+    int32_t gr = nrays;
+    uint32_t head = 0, tail = 0;
+    // this lane's pixel of the generation tile
+    PixState P;
+    P.org = mk3(0.0f, 0.0f, 0.0f); P.pk = 0; P.i = 0;
+    bool pvalid = false;
+    // this lane's ray
+    MarchRay R;
+    R.tm0 = R.tm1 = R.tm2 = R.td0 = R.td1 = R.td2 = 0.0f;
+    R.x = R.y = R.z = 4; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.cw = 0;
+    bool hasray = false, running = false, stopped = false;
+    const uint64_t* grid = A.item0.grid;
+    uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy, glast = A.item0.glast;
+
     while (true) {
-        bool need = valid && !active;
-        uint64_t needm = __ballot(need);
-        uint64_t actm = __ballot(active);
-        if (needm == 0 && actm == 0) break;
-        if (needm != 0 && (actm == 0 || __popcll(needm) >= (uint32_t)C.refill)) {
-            if (need) {
-                PendRay P;
-                if (prepare_next(T, C, S, P, mk3(0.0f, 0.0f, 0.0f))) {
-                    kind = begin_ray(R, P);
-                    active = true;
+        const uint64_t runm = __ballot(running);
+        if (runm == 0 || (uint32_t)__popcll(~runm) >= (uint32_t)A.refill) {
+            // ---- finish ended rays
+            if (hasray && !running) {
+                const uint32_t kind = (R.meta >> 16) & 3u, n = R.meta >> 20;
+                const MarchItem& I = march_item<SINGLE>(A, R.meta & 0xffffu);
+                int32_t term = 0;
+                float t = 0.0f;
+                if (stopped) {
+                    bool oob = (uint32_t)(R.x - 4) > (uint32_t)I.lim[0] || (uint32_t)(R.y - 4) > (uint32_t)I.lim[1] ||
+                               (uint32_t)(R.z - 4) > (uint32_t)I.lim[2];
+                    term = oob ? 2 : 1;
+                    t = best_t_of(R.tm0, R.tm1, R.tm2);
+                }
+                A.res[R.slot] = ray_value((int32_t)kind, term, t, R.maxt);
+                if (A.fres) A.fres[R.slot] = (uint16_t)ray_fetches(term, (int32_t)n);
+                hasray = false;
+                stopped = false;
+            }
+            // ---- generate until the ring holds 64 rays or the work is out
+            while (tail - head < 64u && !gen_done) {
+                f3 wd = mk3(0.0f, 0.0f, 0.0f);
+                if (gr >= nrays) {
+                    // next tile: own band first, then the others
+                    uint32_t tile = 0;
+                    bool got = false;
+                    while (misses < 8u) {
+                        uint32_t t = 0;
+                        if (lane == 0) t = atomicAdd(&A.counters[q], 1u);
+                        t = __builtin_amdgcn_readfirstlane(t);
+                        tile = q * A.band + t;
+                        if (t < A.band && tile < A.ntiles) { got = true; break; }
+                        q = (q + 1u) & 7u;
+                        misses++;
+                    }
+                    if (!got) { gen_done = true; break; }
+                    gtile = tile;
+                    gitem = __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gtile));
+                    const MarchItem& I = march_item<SINGLE>(A, gitem);
+                    uint32_t i = 0;
+                    pvalid = tile_lane_pixel(I, gtile - I.tile0, lane, i) && I.T.mat[i] != 0;
+                    if (pvalid) pixel_setup(I.T, i, P, wd);
+                    gr = 0;
+                }
+                // one batch: ray gr of every valid pixel of the tile
+                const MarchItem& I = march_item<SINGLE>(A, gitem);
+                f3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(1.0f, 1.0f, 1.0f);
+                float maxt = 1.0f;
+                int32_t kind = 0;
+                int32_t cv[3] = {0, 0, 0};
+                bool push = false;
+                const uint32_t slot = ((gtile * (uint32_t)nrays + (uint32_t)gr) << 6) | lane;
+                if (pvalid) {
+                    kind = pixel_ray(I.T, P, wd, gr, o, d, maxt);
+                    if (!fast_ray_ok(o, d)) {
+                        uint32_t k = atomicAdd(&A.counters[8], 1u);
+                        A.deferred[k] = slot;
+                    } else {
+                        ray_origin_voxel(o, cv);
+                        if ((uint32_t)cv[0] <= (uint32_t)I.lim[0] && (uint32_t)cv[1] <= (uint32_t)I.lim[1] &&
+                            (uint32_t)cv[2] <= (uint32_t)I.lim[2]) {
+                            push = true;
+                        } else {
+                            // starts outside: term 2 at step 0
+                            A.res[slot] = ray_value(kind, 2, 0.0f, maxt);
+                            if (A.fres) A.fres[slot] = 0;
+                        }
+                    }
+                }
+                const uint64_t pm = __ballot(push);
+                if (push) ring_put_ray(ring, (tail + (uint32_t)__popcll(pm & lane_lt)) & (kRing - 1), o, d, cv, kind,
+                                       slot, gitem);
+                tail += (uint32_t)__popcll(pm);
+                gr++;
+            }
+            // ---- lanes without a ray take one from the ring
+            const uint64_t needm = __ballot(!hasray);
+            const uint32_t avail = tail - head;
+            if (!hasray) {
+                uint32_t rank = (uint32_t)__popcll(needm & lane_lt);
+                if (rank < avail) {
+                    uint32_t e = (head + rank) & (kRing - 1);
+                    R.tm0 = __uint_as_float(ring[0][e]);
+                    R.tm1 = __uint_as_float(ring[1][e]);
+                    R.tm2 = __uint_as_float(ring[2][e]);
+                    R.td0 = __uint_as_float(ring[3][e]);
+                    R.td1 = __uint_as_float(ring[4][e]);
+                    R.td2 = __uint_as_float(ring[5][e]);
+                    uint32_t xy = ring[6][e], zm = ring[7][e];
+                    R.slot = ring[8][e];
+                    uint32_t item = SINGLE ? 0u : ring[9][e];
+                    R.x = (int32_t)(xy & 0xffffu);
+                    R.y = (int32_t)(xy >> 16);
+                    R.z = (int32_t)(zm & 0xffffu);
+                    R.sx = (zm & 0x10000u) ? -1 : 1;
+                    R.sy = (zm & 0x20000u) ? -1 : 1;
+                    R.sz = (zm & 0x40000u) ? -1 : 1;
+                    uint32_t kind = (zm >> 19) & 3u;
+                    R.meta = (kind << 16) | item;
+                    R.maxt = kind == RAY_SHADOW ? A.maxt[0] : (kind == RAY_AO ? A.maxt[1] : A.maxt[2]);
+                    if (!SINGLE) {
+                        const MarchItem& I = A.items[item];
+                        grid = I.grid;
+                        ggx = (uint32_t)I.gx;
+                        ggxy = (uint32_t)I.gxy;
+                        glast = I.glast;
+                    }
+                    R.cw = grid[grid_word(R.x, R.y, R.z, ggx, ggxy, glast)];
+                    hasray = true;
+                    running = true;
+                    stopped = false;
+                }
+            }
+            head += min((uint32_t)__popcll(needm), avail);
+            if (__ballot(hasray) == 0 && gen_done && head == tail) break;
+        }
+        // ---- march
+#pragma unroll
+        for (int u = 0; u < STEPS; ++u) {
+            if (running) {
+                if ((R.cw >> cell_bit(R.x, R.y, R.z)) & 1ull) {
+                    running = false;
+                    stopped = true;
                 } else {
-                    valid = false;
+                    float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
+                    bool s0 = R.tm0 == tmin, s1 = R.tm1 == tmin, s2 = R.tm2 == tmin;
+                    float bt = tmin;
+                    if ((s0 && s1) || (s2 && (s0 || s1))) bt = (s0 && s1 && s2) ? (tmin + tmin) + tmin : tmin + tmin;
+                    R.x = s0 ? R.x + R.sx : R.x;
+                    R.y = s1 ? R.y + R.sy : R.y;
+                    R.z = s2 ? R.z + R.sz : R.z;
+                    R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
+                    R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
+                    R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
+                    R.meta += 1u << 20;
+                    if (!(R.meta < (256u << 20) && bt < R.maxt)) running = false;
+                    R.cw = grid[grid_word(R.x, R.y, R.z, ggx, ggxy, glast)];
                 }
             }
         }
-        int32_t res = 0;
-        float bt = 0.0f;
-#pragma unroll
-        for (int32_t u = 0; u < 4; ++u) {
-            if (active && res == 0) {
-                float best_t;
-                res = dda_step(V, C, R, best_t);
-                bt = best_t;
-            }
-        }
-        if (res != 0) {
-            ray_finish(S, kind, res == 1, res == 1 ? 1 : (res == 2 ? 2 : 0), R.n, bt, R.maxt);
-            active = false;
-            if (S.r >= nrays) {
-                pixel_write(T, S);
-                valid = false;
-            }
-        }
     }
 }
 
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_dda(OccView V, DdaTraceArgs A) {
-    extern __shared__ uint64_t s_lvl[];
-    for (int32_t l = A.lds_first; l < V.nlevels; ++l) {
-        uint32_t nw = (uint32_t)V.wx[l] * (uint32_t)V.wy[l] * (uint32_t)V.wz[l];
-        for (uint32_t j = threadIdx.x; j < nw; j += BLOCK) s_lvl[A.lds_off[l] + j] = V.lvl[l][j];
-    }
-    __syncthreads();
-    TileCtx C;
-    C.s_lvl = s_lvl;
-    C.lds_first = A.lds_first;
-    C.lds_off[0] = A.lds_off[0]; C.lds_off[1] = A.lds_off[1]; C.lds_off[2] = A.lds_off[2];
-    C.deferred = A.deferred;
-    C.ndeferred = A.ndeferred;
-    C.item = 0;
-    C.refill = A.refill;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t xcd = blockIdx.x & 7u;
-    const uint32_t wave = (blockIdx.x >> 3) * (BLOCK / 64) + (threadIdx.x >> 6);   // wave index within the XCD
-    const uint32_t nwave = (gridDim.x >> 3) * (BLOCK / 64);                         // waves per XCD
-    const uint32_t band0 = xcd * A.band;
-    const uint32_t band1 = min(A.ntiles, band0 + A.band);
-    // The waves of an XCD walk the tiles of its band.
-    for (uint32_t tile = band0 + wave; tile < band1; tile += nwave)
-        trace_tile(V, A.t, C, (tile % A.tiles_x) * 8 + (lane & 7u), (tile / A.tiles_x) * 8 + (lane >> 3));
-}
-
-// Batched trace: many scenes and views in one launch. The tiles of all items
-// form one sequence cut into 8 contiguous bands, one per XCD; a wave takes the
-// next tile of its XCD's band from an atomic counter and moves on to the
-// other bands once its own is exhausted.
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_trace_batch(BatchArgs B) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t q = blockIdx.x & 7u;
-    uint32_t misses = 0;
-    while (misses < 8) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&B.counters[q], 1u);
-        t = __builtin_amdgcn_readfirstlane(t);
-        uint32_t tile = q * B.band + t;
-        if (t >= B.band || tile >= B.ntiles) {
-            q = (q + 1u) & 7u;
-            misses++;
-            continue;
-        }
-        misses = 0;
-        uint32_t lo = 0, hi = B.nitems - 1;          // item with tile0 <= tile
-        while (lo < hi) {
-            uint32_t mid = (lo + hi + 1u) >> 1;
-            if (B.descs[mid].tile0 <= tile) lo = mid;
-            else hi = mid - 1u;
-        }
-        const uint32_t item = __builtin_amdgcn_readfirstlane(lo);
-        const BatchDesc& D = B.descs[item];
-        TileCtx C;
-        C.s_lvl = nullptr;
-        C.lds_first = D.V.nlevels;
-        C.lds_off[0] = C.lds_off[1] = C.lds_off[2] = 0;
-        C.deferred = B.deferred;
-        C.ndeferred = B.ndeferred;
-        C.item = item;
-        C.refill = B.refill;
-        uint32_t lt = tile - D.tile0;
-        trace_tile(D.V, D.T, C, (lt % D.tiles_x) * 8 + (lane & 7u), (lt / D.tiles_x) * 8 + (lane >> 3));
-    }
-}
-
-// Deferred pixels of a batch, traced whole by the literal path.
-__global__ void __launch_bounds__(256) k_trace_list_batch(const BatchDesc* descs, const unsigned long long* list,
-                                                          const uint32_t* count) {
-    uint32_t n = *count;
+// Rays listed by k_march (outside its fast path), one thread each, literal DDA.
+template <bool SINGLE>
+__global__ void __launch_bounds__(256) k_march_rest(MarchArgs A) {
+    const uint32_t n = A.counters[8];
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        unsigned long long e = list[j];
-        const BatchDesc& D = descs[(uint32_t)(e >> 32)];
-        trace_pixel_exact(D.V, D.T, (uint32_t)e);
+        uint32_t slot = A.deferred[j];
+        uint32_t l = slot & 63u, tr = slot >> 6;
+        uint32_t gt = tr / (uint32_t)A.nrays, r = tr % (uint32_t)A.nrays;
+        const MarchItem& I = march_item<SINGLE>(A, item_of_tile<SINGLE>(A, gt));
+        uint32_t i = 0;
+        (void)tile_lane_pixel(I, gt - I.tile0, l, i);
+        PixState P;
+        f3 wd;
+        pixel_setup(I.T, i, P, wd);
+        f3 o, d;
+        float maxt;
+        int32_t kind = pixel_ray(I.T, P, wd, (int32_t)r, o, d, maxt);
+        RayOut R;
+        dda_exact(I.V, o, d, maxt, R);
+        A.res[slot] = ray_value(kind, R.term, R.t, maxt);
+        if (A.fres) A.fres[slot] = (uint16_t)R.fetches;
     }
 }
 
-template <int BLOCK>
-static const void* dda_kernel() {
-    return reinterpret_cast<const void*>(&k_trace_dda<BLOCK>);
-}
-
-const void* trace_dda_kernel(int block) {
-    switch (block) {
-        case 256: return dda_kernel<256>();
-        case 1024: return dda_kernel<1024>();
-        default: return dda_kernel<512>();
+// Per-pixel fold of the ray slots in ray order (LightAmbient.frag:160-175:
+// shadow lit, mean of the squared AO values, spec value) and sky pixels.
+template <bool SINGLE>
+__global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
+    const uint32_t gt = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (gt >= A.ntiles) return;
+    const uint32_t l = threadIdx.x & 63u;
+    const MarchItem& I = march_item<SINGLE>(A, __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gt)));
+    uint32_t i = 0;
+    if (!tile_lane_pixel(I, gt - I.tile0, l, i)) return;
+    const TraceArgs& T = I.T;
+    const vx_trace_params& P = T.params;
+    if (T.mat[i] == 0) {
+        T.shadow[i] = 0; T.ao[i] = 0.0f; T.spec[i] = 0.0f;
+        if (T.fetches) T.fetches[i] = 0;
+        return;
     }
+    uint32_t lit = 1, nf = 0;
+    float aosum = 0.0f, spv = 1.0f;
+    const size_t base = (((size_t)gt * (size_t)A.nrays) << 6) | l;
+    for (int32_t r = 0; r < A.nrays; ++r) {
+        const size_t s = base + ((size_t)r << 6);
+        float v = A.res[s];
+        if (r < P.n_shadow) lit = v != 0.0f ? 1u : 0u;
+        else if (r < P.n_shadow + P.n_ao) aosum = aosum + v * v;
+        else spv = v;
+        if (A.fres) nf += A.fres[s];
+    }
+    T.shadow[i] = (uint8_t)lit;
+    T.ao[i] = P.n_ao > 0 ? aosum / (float)P.n_ao : 1.0f;
+    T.spec[i] = P.n_spec > 0 ? spv : 1.0f;
+    if (T.fetches) T.fetches[i] = nf;
 }
 
-hipError_t launch_trace_dda_kernel(int block, uint32_t grid, size_t lds, hipStream_t stream, const OccView& V,
-                                   const DdaTraceArgs& A) {
-    switch (block) {
-        case 256: hipLaunchKernelGGL(k_trace_dda<256>, dim3(grid), dim3(256), lds, stream, V, A); break;
-        case 1024: hipLaunchKernelGGL(k_trace_dda<1024>, dim3(grid), dim3(1024), lds, stream, V, A); break;
-        default: hipLaunchKernelGGL(k_trace_dda<512>, dim3(grid), dim3(512), lds, stream, V, A); break;
+// Launchers (template instances live in this file).
+template <int BLOCK, int STEPS>
+static void launch_march_t(bool single, uint32_t grid, hipStream_t s, const MarchArgs& A) {
+    if (single) hipLaunchKernelGGL((k_march<BLOCK, STEPS, true>), dim3(grid), dim3(BLOCK), 0, s, A);
+    else hipLaunchKernelGGL((k_march<BLOCK, STEPS, false>), dim3(grid), dim3(BLOCK), 0, s, A);
+}
+
+template <int BLOCK, int STEPS>
+static const void* march_kernel_t(bool single) {
+    return single ? reinterpret_cast<const void*>(&k_march<BLOCK, STEPS, true>)
+                  : reinterpret_cast<const void*>(&k_march<BLOCK, STEPS, false>);
+}
+
+const void* march_kernel(int block, int steps, bool single) {
+    if (block == 256) {
+        if (steps == 2) return march_kernel_t<256, 2>(single);
+        if (steps == 8) return march_kernel_t<256, 8>(single);
+        return march_kernel_t<256, 4>(single);
+    }
+    if (steps == 2) return march_kernel_t<512, 2>(single);
+    if (steps == 8) return march_kernel_t<512, 8>(single);
+    return march_kernel_t<512, 4>(single);
+}
+
+hipError_t launch_march(int block, int steps, bool single, uint32_t grid, uint32_t rest_grid, hipStream_t s,
+                        const MarchArgs& A, hipEvent_t e0, hipEvent_t e1) {
+    if (e0) {
+        hipError_t er = hipEventRecord(e0, s);
+        if (er != hipSuccess) return er;
+    }
+    if (block == 256) {
+        if (steps == 2) launch_march_t<256, 2>(single, grid, s, A);
+        else if (steps == 8) launch_march_t<256, 8>(single, grid, s, A);
+        else launch_march_t<256, 4>(single, grid, s, A);
+    } else {
+        if (steps == 2) launch_march_t<512, 2>(single, grid, s, A);
+        else if (steps == 8) launch_march_t<512, 8>(single, grid, s, A);
+        else launch_march_t<512, 4>(single, grid, s, A);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (e1) {
+        e = hipEventRecord(e1, s);
+        if (e != hipSuccess) return e;
+    }
+    uint32_t rgrid = (A.ntiles + 3u) / 4u;
+    if (single) {
+        hipLaunchKernelGGL(k_march_rest<true>, dim3(rest_grid), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_resolve<true>, dim3(rgrid), dim3(256), 0, s, A);
+    } else {
+        hipLaunchKernelGGL(k_march_rest<false>, dim3(rest_grid), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_resolve<false>, dim3(rgrid), dim3(256), 0, s, A);
     }
     return hipGetLastError();
 }
 
-const void* trace_batch_kernel(int block) {
-    switch (block) {
-        case 256: return reinterpret_cast<const void*>(&k_trace_batch<256>);
-        case 1024: return reinterpret_cast<const void*>(&k_trace_batch<1024>);
-        default: return reinterpret_cast<const void*>(&k_trace_batch<512>);
+// March grid words for grid cells [g0, g0 + n) (grid coordinates, border
+// cells included): the occupancy cell plus the out-of-bounds voxel bits.
+__global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32_t g0x, int32_t g0y, int32_t g0z,
+                        int32_t nx, int32_t ny, int32_t nz) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nx * ny * nz) return;
+    int32_t cx = g0x + (int32_t)(idx % (size_t)nx);
+    int32_t cy = g0y + (int32_t)((idx / (size_t)nx) % (size_t)ny);
+    int32_t cz = g0z + (int32_t)(idx / ((size_t)nx * ny));
+    const int32_t f0[3] = {4 * (cx - 1), 4 * (cy - 1), 4 * (cz - 1)};
+    const int32_t lim[3] = {2 * V.sx + 1, 2 * V.sy + 1, 2 * V.sz + 1};
+    uint32_t om[3];
+    bool any_in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        om[k] = 0;
+        for (int p = 0; p < 4; ++p)
+            if ((uint32_t)(f0[k] + p) > (uint32_t)lim[k]) om[k] |= 1u << p;
+        any_in = any_in && om[k] != 15u;
     }
-}
-
-hipError_t launch_trace_batch_kernel(int block, uint32_t grid, hipStream_t stream, const BatchArgs& B) {
-    switch (block) {
-        case 256: hipLaunchKernelGGL(k_trace_batch<256>, dim3(grid), dim3(256), 0, stream, B); break;
-        case 1024: hipLaunchKernelGGL(k_trace_batch<1024>, dim3(grid), dim3(1024), 0, stream, B); break;
-        default: hipLaunchKernelGGL(k_trace_batch<512>, dim3(grid), dim3(512), 0, stream, B); break;
+    uint64_t w = 0;
+    if (any_in) w = V.cells[cell_index(V, f0[0], f0[1], f0[2])];
+    for (uint32_t b = 0; b < 64; ++b) {
+        uint32_t px = b & 3u, py = (b >> 2) & 3u, pz = b >> 4;
+        if (((om[0] >> px) | (om[1] >> py) | (om[2] >> pz)) & 1u) w |= 1ull << b;
     }
-    return hipGetLastError();
+    grid[((size_t)cz * gy + cy) * gx + cx] = w;
 }
 
 // ===========================================================================
