@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvx_amd.so")
+# VX_AMD_LIB selects another build of the library (same ABI).
+LIB_PATH = os.environ.get("VX_AMD_LIB") or os.path.join(HERE, "libvx_amd.so")
 
 VX_OK = 0
 VX_ERR_INVALID_ARGUMENT = -1

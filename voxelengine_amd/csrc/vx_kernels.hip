@@ -576,7 +576,7 @@ VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
 // set. A ray that starts inside the bounds stops at its first set bit, either
 // an occupied voxel (hit) or the first out-of-bounds voxel (term 2), told
 // apart once at its end, and every voxel it visits lies inside the grid.
-// Lane coordinates carry a +4 bias, so grid cell = biased coordinate >> 2.
+// Lane coordinates carry a +4 bias (grid cell = (X + 4) >> 2 per axis).
 // ===========================================================================
 
 constexpr int kRing = 128;       // ring entries per wave
@@ -760,20 +760,34 @@ VXD void ring_put_ray(uint32_t (*ring)[kRing], uint32_t e, f3 o, f3 d, const int
     ring[9][e] = item;
 }
 
-// The ray a lane marches: coordinates biased by +4; meta = n << 20 | kind << 16
-// | item, so n++ is meta += 1 << 20 and n < 256 is meta < 256 << 20.
+// The ray a lane marches (coordinates as below); meta = n << 20 | kind << 16 |
+// item, so n++ is meta += 1 << 20 and n < 256 is meta < 256 << 20.
 struct MarchRay {
     float tm0, tm1, tm2, td0, td1, td2;
     int32_t x, y, z, sx, sy, sz;
     uint32_t meta;
     float maxt;
     uint32_t slot;
+    uint32_t ci;           // grid word held in cw
     uint64_t cw;
 };
 
-// Grid word index of biased voxel (x, y, z), clamped into the grid.
+// Lane coordinates: x = X + 4, y = (Y + 4) << 2, z = (Z + 4) << 4 for fine
+// voxel (X, Y, Z), so the in-cell fields sit at their cell-bit positions.
+
+// Shift amount selecting the lane's voxel in its cell word: the low 6 bits are
+// cell_bit(X, Y, Z) = (x & 3) | (y & 12) | (z & 48); the bits above are
+// don't-care, since a 64-bit shift reads only the low 6 bits of its amount.
+VXD uint32_t cell_shift(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t t, r;
+    asm("v_bfi_b32 %0, 12, %1, %2" : "=v"(t) : "v"(y), "v"(z));
+    asm("v_bfi_b32 %0, 3, %1, %2" : "=v"(r) : "v"(x), "v"(t));
+    return r;
+}
+
+// Grid word index of the lane's voxel, clamped into the grid.
 VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gxy, uint32_t glast) {
-    uint32_t ci = __umul24((uint32_t)(z >> 2), gxy) + __umul24((uint32_t)(y >> 2), gx) + (uint32_t)(x >> 2);
+    uint32_t ci = __umul24((uint32_t)(z >> 6), gxy) + __umul24((uint32_t)(y >> 4), gx) + (uint32_t)(x >> 2);
     return min(ci, glast);
 }
 
@@ -799,7 +813,7 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
     // this lane's ray
     MarchRay R;
     R.tm0 = R.tm1 = R.tm2 = R.td0 = R.td1 = R.td2 = 0.0f;
-    R.x = R.y = R.z = 4; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.cw = 0;
+    R.x = R.y = R.z = 4; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.ci = 0; R.cw = 0;
     bool hasray = false, running = false, stopped = false;
     const uint64_t* grid = A.item0.grid;
     uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy, glast = A.item0.glast;
@@ -814,8 +828,9 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
                 int32_t term = 0;
                 float t = 0.0f;
                 if (stopped) {
-                    bool oob = (uint32_t)(R.x - 4) > (uint32_t)I.lim[0] || (uint32_t)(R.y - 4) > (uint32_t)I.lim[1] ||
-                               (uint32_t)(R.z - 4) > (uint32_t)I.lim[2];
+                    bool oob = (uint32_t)(R.x - 4) > (uint32_t)I.lim[0] ||
+                               (uint32_t)((R.y >> 2) - 4) > (uint32_t)I.lim[1] ||
+                               (uint32_t)((R.z >> 4) - 4) > (uint32_t)I.lim[2];
                     term = oob ? 2 : 1;
                     t = best_t_of(R.tm0, R.tm1, R.tm2);
                 }
@@ -897,11 +912,11 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
                     R.slot = ring[8][e];
                     uint32_t item = SINGLE ? 0u : ring[9][e];
                     R.x = (int32_t)(xy & 0xffffu);
-                    R.y = (int32_t)(xy >> 16);
-                    R.z = (int32_t)(zm & 0xffffu);
+                    R.y = (int32_t)(xy >> 16) << 2;
+                    R.z = (int32_t)(zm & 0xffffu) << 4;
                     R.sx = (zm & 0x10000u) ? -1 : 1;
-                    R.sy = (zm & 0x20000u) ? -1 : 1;
-                    R.sz = (zm & 0x40000u) ? -1 : 1;
+                    R.sy = (zm & 0x20000u) ? -4 : 4;
+                    R.sz = (zm & 0x40000u) ? -16 : 16;
                     uint32_t kind = (zm >> 19) & 3u;
                     R.meta = (kind << 16) | item;
                     R.maxt = kind == RAY_SHADOW ? A.maxt[0] : (kind == RAY_AO ? A.maxt[1] : A.maxt[2]);
@@ -912,7 +927,8 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
                         ggxy = (uint32_t)I.gxy;
                         glast = I.glast;
                     }
-                    R.cw = grid[grid_word(R.x, R.y, R.z, ggx, ggxy, glast)];
+                    R.ci = grid_word(R.x, R.y, R.z, ggx, ggxy, glast);
+                    R.cw = grid[R.ci];
                     hasray = true;
                     running = true;
                     stopped = false;
@@ -925,7 +941,7 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
 #pragma unroll
         for (int u = 0; u < STEPS; ++u) {
             if (running) {
-                if ((R.cw >> cell_bit(R.x, R.y, R.z)) & 1ull) {
+                if ((R.cw >> (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) & 1ull) {
                     running = false;
                     stopped = true;
                 } else {
@@ -941,7 +957,12 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
                     R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
                     R.meta += 1u << 20;
                     if (!(R.meta < (256u << 20) && bt < R.maxt)) running = false;
-                    R.cw = grid[grid_word(R.x, R.y, R.z, ggx, ggxy, glast)];
+                    // the word is fetched only when the ray enters another cell
+                    const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy, glast);
+                    if (ci != R.ci) {
+                        R.ci = ci;
+                        R.cw = grid[ci];
+                    }
                 }
             }
         }
