@@ -45,7 +45,8 @@ def measured_traffic():
 
 def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
     """Oracle (C restatement, OpenMP over pixels) on a bounded, deterministic
-    pixel subsample: every k-th hit pixel, sized from a pilot run to ~budget_s."""
+    sample of the same frame: every k-th hit pixel, sized from a pilot run to
+    about budget_s; when the whole frame takes less, the whole frame repeated."""
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0))
     hit = np.nonzero(mat > 0)[0].astype(np.int32)
@@ -56,12 +57,16 @@ def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
     n_target = int(min(hit.size, max(pilot.size, pilot.size * budget_s / max(dt, 1e-6))))
     stride = max(1, hit.size // n_target)
     sample = np.ascontiguousarray(hit[::stride])
-    t0 = time.perf_counter()
-    O.trace(occ_host, vd, params, pos, nrm, mat, pal, pixels=sample, nthreads=cores)
-    dt = time.perf_counter() - t0
-    rays = sample.size * (params["n_shadow"] + params["n_ao"] + params["n_spec"])
-    return {"value": rays / dt / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
-            "sample": f"every {stride}th hit pixel ({sample.size} px, {rays} rays, {dt:.1f} s)"}
+    reps, t_total = 0, 0.0
+    while reps < 64 and (reps == 0 or t_total < budget_s):
+        t0 = time.perf_counter()
+        O.trace(occ_host, vd, params, pos, nrm, mat, pal, pixels=sample, nthreads=cores)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    rays = reps * sample.size * (params["n_shadow"] + params["n_ao"] + params["n_spec"])
+    what = "the whole frame" if stride == 1 else f"every {stride}th hit pixel"
+    return {"value": rays / t_total / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
+            "sample": f"{what} ({sample.size} px) x {reps}: {rays} rays in {t_total:.1f} s"}
 
 
 def bench_batched(ctx, pal_d, reps=5):
@@ -124,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the headline trace (for profiling)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed region, copy 1 GiB device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
@@ -203,6 +210,13 @@ def main():
     value = rays_per_frame / (ms_per_step * 1e-3) / 1e9
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = measured_traffic()
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     out = {
         "metric": "Grays/s (shadow+AO+spec-occl) at 1080p/512^3",

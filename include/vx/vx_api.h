@@ -153,7 +153,7 @@ typedef struct {
     int32_t block;           /* ray-parallel kernel threads per block: 256 or 512 (0 = 256) */
     int32_t blocks_per_cu;   /* persistent grid = CUs x this (0 = occupancy limit) */
     int32_t refill;          /* lanes without a running ray that trigger a refill, 1..64 (0 = 16) */
-    int32_t steps;           /* DDA steps between refill checks: 2, 4 or 8 (0 = 4) */
+    int32_t steps;           /* DDA steps between refill checks: 2, 4 or 8 (0 = 8) */
     int32_t pad[3];
 } vx_trace_tuning;
 

@@ -889,7 +889,7 @@ struct MarchShape {
 static vx_status march_shape(vx_ctx* ctx, bool single, MarchShape& sh) {
     const vx_trace_tuning& tu = ctx->tuning;
     sh.block = tu.block == 512 ? 512 : 256;
-    sh.steps = tu.steps ? tu.steps : 4;
+    sh.steps = tu.steps ? tu.steps : 8;
     sh.rest_grid = 64;
     return persistent_grid(ctx, march_kernel(sh.block, sh.steps, single), sh.block, 0, tu.blocks_per_cu, sh.grid);
 }
