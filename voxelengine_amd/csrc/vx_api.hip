@@ -66,7 +66,7 @@ struct MarchBuffers {
     float* res;
     uint16_t* fres;
     uint32_t* deferred;
-    uint32_t* counters;      // 9
+    uint32_t* counters;      // kCounterWords
     size_t slots;            // capacity of res / deferred
     size_t fslots;           // capacity of fres
 };
@@ -221,7 +221,7 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     std::memset(&c->tuning, 0, sizeof(c->tuning));
     c->profile = false;
     hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
-    if (me == hipSuccess) me = hipMalloc(&c->march.counters, 9 * sizeof(uint32_t));
+    if (me == hipSuccess) me = hipMalloc(&c->march.counters, kCounterWords * sizeof(uint32_t));
     if (me != hipSuccess) {
         (void)hipFree(c->grad_acc);
         march_buffers_free(c->march);
@@ -861,7 +861,7 @@ static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
         VX_HIP(hipMalloc(&m.fres, slots * sizeof(uint16_t)));
         m.fslots = slots;
     }
-    if (!m.counters) VX_HIP(hipMalloc(&m.counters, 9 * sizeof(uint32_t)));
+    if (!m.counters) VX_HIP(hipMalloc(&m.counters, kCounterWords * sizeof(uint32_t)));
     return VX_OK;
 }
 
@@ -916,6 +916,7 @@ static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tra
     if (vx_status st = ensure_march(ctx->march, slots, T.fetches != nullptr)) return st;
     A.counters = ctx->march.counters;
     A.deferred = ctx->march.deferred;
+    A.deferred_cap = (uint32_t)slots;
     A.res = ctx->march.res;
     A.fres = T.fetches ? ctx->march.fres : nullptr;
     MarchShape sh;
@@ -929,7 +930,7 @@ static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tra
         }
         ctx->prof.emplace_back(e0, e1);
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, A.counters, 9);
+    hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(kCounterWords, 256)), dim3(256), 0, stream, A.counters, (int)kCounterWords);
     hipError_t e = launch_march(sh.block, sh.steps, true, sh.grid, sh.rest_grid, stream, A, e0, e1);
     if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of the ray-parallel trace failed: %s", hipGetErrorString(e));
     return VX_OK;
@@ -1060,7 +1061,7 @@ static void batch_free(vx_batch* b) {
 // The kernels of one batched launch: clear the queue counters (a fill kernel,
 // no memset node), march, listed rays, resolve.
 static hipError_t batch_enqueue(vx_batch* b, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, b->march.counters, 9);
+    hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(kCounterWords, 256)), dim3(256), 0, s, b->march.counters, (int)kCounterWords);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_march(b->shape.block, b->shape.steps, false, b->shape.grid, b->shape.rest_grid, s, b->args, e0,
@@ -1128,6 +1129,7 @@ extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, ui
     A.band = cdiv(tiles, 8);
     A.counters = b->march.counters;
     A.deferred = b->march.deferred;
+    A.deferred_cap = (uint32_t)slots;
     A.res = b->march.res;
     A.fres = fetches ? b->march.fres : nullptr;
     if (vx_status st = march_shape(ctx, false, b->shape)) {

@@ -48,13 +48,19 @@ struct MarchItem {
     uint32_t tile0;            // first global tile
 };
 
+// Queue counters of the ray-parallel trace, one 128-byte line each: 8 band
+// counters, then the listed-ray count.
+constexpr uint32_t kCounterStride = 32;
+constexpr uint32_t kCounterWords = 9 * kCounterStride;
+
 struct MarchArgs {
     MarchItem item0;           // the item of a single-view launch (a copy of items[0] otherwise)
     const MarchItem* items;    // device array of a batched launch
     uint32_t nitems, ntiles;   // items, global tiles
     uint32_t band;             // tiles per XCD band
-    uint32_t* counters;        // [0..7] band counters, [8] listed-ray count
+    uint32_t* counters;        // band counters, listed-ray count (kCounterStride apart)
     uint32_t* deferred;        // slots of rays left to the literal path
+    uint32_t deferred_cap;     // capacity of deferred (every slot fits)
     float* res;                // per-ray value, slot = ((tile * nrays + r) << 6) | pixel-in-tile
     uint16_t* fres;            // per-ray fetch count, or null
     int32_t nrays;

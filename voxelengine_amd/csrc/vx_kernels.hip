@@ -791,8 +791,11 @@ VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gx
     return min(ci, glast);
 }
 
+// Occupancy: 7 waves per SIMD for a single view (72 VGPRs, 140 KB of LDS rings
+// per CU), 6 for batches, whose lanes also hold per-item grid addresses.
 template <int BLOCK, int STEPS, bool SINGLE>
-__global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SINGLE ? 7 : 5, 8)))
+k_march(MarchArgs A) {
     constexpr int WAVES = BLOCK / 64;
     __shared__ uint32_t ring_s[WAVES][kRingFields][kRing];
     const uint32_t lane = threadIdx.x & 63u;
@@ -848,7 +851,7 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
                     bool got = false;
                     while (misses < 8u) {
                         uint32_t t = 0;
-                        if (lane == 0) t = atomicAdd(&A.counters[q], 1u);
+                        if (lane == 0) t = atomicAdd(&A.counters[q * kCounterStride], 1u);
                         t = __builtin_amdgcn_readfirstlane(t);
                         tile = q * A.band + t;
                         if (t < A.band && tile < A.ntiles) { got = true; break; }
@@ -875,7 +878,7 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
                 if (pvalid) {
                     kind = pixel_ray(I.T, P, wd, gr, o, d, maxt);
                     if (!fast_ray_ok(o, d)) {
-                        uint32_t k = atomicAdd(&A.counters[8], 1u);
+                        uint32_t k = atomicAdd(&A.counters[8 * kCounterStride], 1u);
                         A.deferred[k] = slot;
                     } else {
                         ray_origin_voxel(o, cv);
@@ -972,7 +975,7 @@ __global__ void __launch_bounds__(BLOCK) k_march(MarchArgs A) {
 // Rays listed by k_march (outside its fast path), one thread each, literal DDA.
 template <bool SINGLE>
 __global__ void __launch_bounds__(256) k_march_rest(MarchArgs A) {
-    const uint32_t n = A.counters[8];
+    const uint32_t n = min(A.counters[8 * kCounterStride], A.deferred_cap);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         uint32_t slot = A.deferred[j];
         uint32_t l = slot & 63u, tr = slot >> 6;
