@@ -873,7 +873,6 @@ static MarchItem make_march_item(const vx_volume* vol, const TraceArgs& T, uint3
     I.grid = vol->grid;
     I.gx = vol->gx;
     I.gxy = vol->gx * vol->gy;
-    I.glast = (uint32_t)(vol->ngrid - 1);
     I.lim[0] = 2 * vol->sx + 1; I.lim[1] = 2 * vol->sy + 1; I.lim[2] = 2 * vol->sz + 1;
     I.tiles_x = cdiv((uint32_t)T.view.width, 8);
     I.ntiles = I.tiles_x * cdiv((uint32_t)T.view.height, 8);

@@ -40,9 +40,8 @@ struct TraceArgs {
 struct MarchItem {
     OccView V;                 // occupancy (literal path of listed rays)
     TraceArgs T;               // view, params, G-buffer, outputs
-    const uint64_t* grid;      // march grid (border cells included)
+    const uint64_t* grid;      // march grid (border cells included, words bit-reversed)
     int32_t gx, gxy;           // march grid strides in words
-    uint32_t glast;            // last march grid word
     int32_t lim[3];            // 2*s+1: last in-bounds fine voxel per axis
     uint32_t tiles_x, ntiles;  // 8x8 tiles of the view
     uint32_t tile0;            // first global tile

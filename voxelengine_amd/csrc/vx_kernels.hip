@@ -785,10 +785,10 @@ VXD uint32_t cell_shift(uint32_t x, uint32_t y, uint32_t z) {
     return r;
 }
 
-// Grid word index of the lane's voxel, clamped into the grid.
-VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gxy, uint32_t glast) {
-    uint32_t ci = __umul24((uint32_t)(z >> 6), gxy) + __umul24((uint32_t)(y >> 4), gx) + (uint32_t)(x >> 2);
-    return min(ci, glast);
+// Grid word index of the lane's voxel. It stays inside the grid: a ray starts
+// inside the bounds and stops at the first border voxel it reaches.
+VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gxy) {
+    return __umul24((uint32_t)(z >> 6), gxy) + (__umul24((uint32_t)(y >> 4), gx) + (uint32_t)(x >> 2));
 }
 
 // Occupancy: 7 waves per SIMD for a single view (72 VGPRs, 140 KB of LDS rings
@@ -819,7 +819,7 @@ k_march(MarchArgs A) {
     R.x = R.y = R.z = 4; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.ci = 0; R.cw = 0;
     bool hasray = false, running = false, stopped = false;
     const uint64_t* grid = A.item0.grid;
-    uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy, glast = A.item0.glast;
+    uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy;
 
     while (true) {
         const uint64_t runm = __ballot(running);
@@ -928,9 +928,8 @@ k_march(MarchArgs A) {
                         grid = I.grid;
                         ggx = (uint32_t)I.gx;
                         ggxy = (uint32_t)I.gxy;
-                        glast = I.glast;
                     }
-                    R.ci = grid_word(R.x, R.y, R.z, ggx, ggxy, glast);
+                    R.ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
                     R.cw = grid[R.ci];
                     hasray = true;
                     running = true;
@@ -944,7 +943,8 @@ k_march(MarchArgs A) {
 #pragma unroll
         for (int u = 0; u < STEPS; ++u) {
             if (running) {
-                if ((R.cw >> (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) & 1ull) {
+                // grid words are bit-reversed: voxel bit b sits at bit 63 - b
+                if ((int64_t)(R.cw << (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) < 0) {
                     running = false;
                     stopped = true;
                 } else {
@@ -961,7 +961,7 @@ k_march(MarchArgs A) {
                     R.meta += 1u << 20;
                     if (!(R.meta < (256u << 20) && bt < R.maxt)) running = false;
                     // the word is fetched only when the ray enters another cell
-                    const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy, glast);
+                    const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
                     if (ci != R.ci) {
                         R.ci = ci;
                         R.cw = grid[ci];
@@ -1087,7 +1087,9 @@ hipError_t launch_march(int block, int steps, bool single, uint32_t grid, uint32
 }
 
 // March grid words for grid cells [g0, g0 + n) (grid coordinates, border
-// cells included): the occupancy cell plus the out-of-bounds voxel bits.
+// cells included): the occupancy cell plus the out-of-bounds voxel bits, bit
+// reversed (voxel bit b stored at bit 63 - b, so a left shift by b moves it to
+// the sign bit).
 __global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32_t g0x, int32_t g0y, int32_t g0z,
                         int32_t nx, int32_t ny, int32_t nz) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1112,7 +1114,7 @@ __global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32
         uint32_t px = b & 3u, py = (b >> 2) & 3u, pz = b >> 4;
         if (((om[0] >> px) | (om[1] >> py) | (om[2] >> pz)) & 1u) w |= 1ull << b;
     }
-    grid[((size_t)cz * gy + cy) * gx + cx] = w;
+    grid[((size_t)cz * gy + cy) * gx + cx] = __builtin_bitreverse64(w);
 }
 
 // ===========================================================================
