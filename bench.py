@@ -122,6 +122,67 @@ def bench_batched(ctx, pal_d, reps=5):
     return res
 
 
+def bench_large(ctx, pal_d, reps=3):
+    """Config L: 1024^3 city blocks (20000 grounded boxes), 3 levels, four
+    3840x2160 views with 1 shadow + 32 AO + 1 spec ray per hit pixel in one
+    graph-captured launch; palette and light gradients of all four views
+    accumulated in one adjoint launch."""
+    import torch
+    from voxelengine_amd import api, configs, views
+    cfg = configs.CONFIGS["L"]
+    nx, ny, nz = cfg["dims"]
+    W, H = cfg["width"], cfg["height"]
+    params = configs.trace_params(**cfg["params"])
+    m = api.Model(ctx, nx, ny, nz)
+    m.generate(seed=configs.SEED, terrain_fill=cfg["fill"], nboxes=cfg["nboxes"], box_min=cfg["box"][0],
+               box_max=cfg["box"][1], grounded=cfg["grounded"])
+    m.build_mips()
+    occ = api.Volume(ctx, nx // 2, ny // 2, nz // 2, nlevels=cfg["levels"])
+    occ.clear()
+    occ.rebuild([api.make_instance(m.mip_ptr(0), m.dims, api.IDENTITY, api.IDENTITY)])
+    occ.build_levels()
+    items = []
+    for k in range(cfg["nviews"]):
+        vd = views.scene_view(cfg["dims"], W, H, variant=k)
+        g = api.GBuffer(W, H)
+        api.primary(ctx, m, vd, g)
+        items.append((occ, vd, g, api.Visibility(W, H)))
+    torch.cuda.synchronize()
+    hit = sum(int((it[2].material > 0).sum().item()) for it in items)
+    rays = hit * configs.rays_per_pixel(params)
+    batch = api.Batch(ctx, items, pal_d, params)
+    batch.launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        batch.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    light = configs.default_light()
+    gen = torch.Generator(device="cuda").manual_seed(configs.SEED)
+    sitems = [(vd, g, vis, torch.randn((W * H, 3), generator=gen, device="cuda"), None) for (_, vd, g, vis) in items]
+    dpal = torch.zeros((256, 6), dtype=torch.float32, device="cuda")
+    dlight = torch.zeros(6, dtype=torch.float32, device="cuda")
+    api.shade_backward_batch(ctx, sitems, pal_d, light, dpal, dlight)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        api.shade_backward_batch(ctx, sitems, pal_d, light, dpal, dlight)
+    e1.record()
+    torch.cuda.synchronize()
+    bwd_ms = e0.elapsed_time(e1) / 5
+    batch.close()
+    out = {"views": cfg["nviews"], "resolution": f"{W}x{H}", "hit_pixels": hit, "rays": rays,
+           "forward_ms": round(fwd_ms, 3), "forward_grays_per_s": round(rays / (fwd_ms * 1e-3) / 1e9, 3),
+           "backward_ms": round(bwd_ms, 4), "backward_grays_per_s": round(rays / (bwd_ms * 1e-3) / 1e9, 2),
+           "backward_px_per_s": round(hit / (bwd_ms * 1e-3), 1)}
+    del items, sitems, occ, m
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +337,8 @@ def main():
 
     if not args.no_extras:
         out["batched"] = bench_batched(ctx, pal_d)
+        log("config L ...")
+        out["large_batched"] = bench_large(ctx, pal_d)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

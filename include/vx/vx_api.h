@@ -269,6 +269,25 @@ vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbu
                             const float light[6], const float* dl_drgb_dev, float* dpalette_dev,
                             float* dlight_dev, void* stream);
 
+/* One view of a batched adjoint: its G-buffer, fixed visibility, optional
+ * SSAO and the loss gradient w.r.t. its shaded image (all device). */
+typedef struct {
+    vx_view view;
+    vx_gbuf gbuf;
+    vx_trace_out vis;
+    const float* ssao;
+    const float* dl_drgb;
+} vx_shade_item;
+
+/* Adjoint over n views sharing one palette and light, gradients accumulated
+ * across the views in one launch (the views of a scene rendered from several
+ * cameras; no counterpart in the reference, which has no adjoint). Same
+ * per-pixel partials as vx_shade_backward, summed exactly in 2^-30 fixed
+ * point, so the result does not depend on the order of views or pixels. */
+vx_status vx_shade_backward_batch(vx_ctx* ctx, const vx_shade_item* items, uint32_t n,
+                                  const float* palette_dev, const float light[6], float* dpalette_dev,
+                                  float* dlight_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

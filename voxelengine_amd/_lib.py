@@ -74,6 +74,10 @@ class BatchItem(C.Structure):
     _fields_ = [("volume", P), ("view", View), ("gbuf", GBuf), ("out", TraceOut)]
 
 
+class ShadeItem(C.Structure):
+    _fields_ = [("view", View), ("gbuf", GBuf), ("vis", TraceOut), ("ssao", P), ("dl_drgb", P)]
+
+
 class TraceTuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
                 ("steps", C.c_int32), ("pad", C.c_int32 * 3)]
@@ -119,6 +123,8 @@ SIGNATURES = {
                           P, P], C.c_int32),
     "vx_shade_backward": ([P, C.POINTER(View), C.POINTER(GBuf), C.POINTER(TraceOut), P, P,
                            C.POINTER(C.c_float), P, P, P, P], C.c_int32),
+    "vx_shade_backward_batch": ([P, C.POINTER(ShadeItem), C.c_uint32, P, C.POINTER(C.c_float), P, P, P],
+                                C.c_int32),
 }
 
 _lib = None

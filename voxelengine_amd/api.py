@@ -320,3 +320,13 @@ def shade_backward(ctx, vd, gbuf, vis, palette, light, dl_drgb, dpalette, dlight
     v = view_struct(vd); g = gbuf.struct(); o = vis.struct()
     check(L.lib().vx_shade_backward(ctx.h, C.byref(v), C.byref(g), C.byref(o), _ptr(ssao), _ptr(palette),
                                     _light(light), _ptr(dl_drgb), _ptr(dpalette), _ptr(dlight), _stream(stream)))
+
+
+def shade_backward_batch(ctx, items, palette, light, dpalette, dlight, stream=None):
+    """items: [(vd, gbuf, vis, dl_drgb, ssao or None)]; gradients of all views
+    accumulated in one launch."""
+    arr = (L.ShadeItem * len(items))()
+    for k, (vd, gbuf, vis, dl, ssao) in enumerate(items):
+        arr[k] = L.ShadeItem(view_struct(vd), gbuf.struct(), vis.struct(), _ptr(ssao), _ptr(dl))
+    check(L.lib().vx_shade_backward_batch(ctx.h, arr, len(items), _ptr(palette), _light(light), _ptr(dpalette),
+                                          _ptr(dlight), _stream(stream)))

@@ -82,6 +82,9 @@ static void march_buffers_free(MarchBuffers& m) {
 struct vx_ctx {
     int32_t device;
     long long* grad_acc;     // 256*6+6 fixed-point gradient accumulator
+    ShadeArgs* shade_items;  // batched adjoint: per-view arguments (device)
+    uint32_t* shade_block0;  // and each view's first block (n + 1 entries)
+    uint32_t shade_cap;
     MarchBuffers march;
     vx_trace_tuning tuning;
     bool profile;            // record events around march kernels
@@ -217,6 +220,9 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     if (!c) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
     c->grad_acc = nullptr;
+    c->shade_items = nullptr;
+    c->shade_block0 = nullptr;
+    c->shade_cap = 0;
     std::memset(&c->march, 0, sizeof(c->march));
     std::memset(&c->tuning, 0, sizeof(c->tuning));
     c->profile = false;
@@ -236,6 +242,8 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     if (!ctx) return VX_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipFree(ctx->grad_acc);
+    (void)hipFree(ctx->shade_items);
+    (void)hipFree(ctx->shade_block0);
     march_buffers_free(ctx->march);
     profile_clear(ctx);
     delete ctx;
@@ -1028,6 +1036,49 @@ extern "C" vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const v
     hipLaunchKernelGGL(k_shade_bwd_finish, dim3(cdiv(nacc, 256)), dim3(256), 0, S(stream), ctx->grad_acc, dpalette,
                        dlight);
     VX_LAUNCHED("k_shade_bwd");
+    return VX_OK;
+}
+
+extern "C" vx_status vx_shade_backward_batch(vx_ctx* ctx, const vx_shade_item* items, uint32_t n,
+                                             const float* palette, const float light[6], float* dpalette,
+                                             float* dlight, void* stream) {
+    VX_REQUIRE(ctx && dpalette && dlight, "ctx/dpalette/dlight is NULL");
+    VX_REQUIRE(n >= 1 && n <= 65536 && items, "batched adjoint needs 1..65536 views");
+    std::vector<ShadeArgs> h(n);
+    std::vector<uint32_t> b0(n + 1);
+    uint64_t blocks = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const vx_shade_item& it = items[i];
+        VX_REQUIRE(it.dl_drgb, "item %u: dl_drgb is NULL", i);
+        if (vx_status st = shade_args(&it.view, &it.gbuf, &it.vis, it.ssao, palette, light, h[i])) return st;
+        h[i].dl_drgb = it.dl_drgb;
+        b0[i] = (uint32_t)blocks;
+        blocks += std::min<uint32_t>(cdiv(h[i].npx, 256), 1024);
+    }
+    b0[n] = (uint32_t)blocks;
+    if (vx_status st = use_device(ctx)) return st;
+    if (ctx->shade_cap < n) {
+        VX_HIP(hipDeviceSynchronize());
+        (void)hipFree(ctx->shade_items);
+        (void)hipFree(ctx->shade_block0);
+        ctx->shade_items = nullptr;
+        ctx->shade_block0 = nullptr;
+        ctx->shade_cap = 0;
+        VX_HIP(hipMalloc(&ctx->shade_items, n * sizeof(ShadeArgs)));
+        VX_HIP(hipMalloc(&ctx->shade_block0, (n + 1) * sizeof(uint32_t)));
+        ctx->shade_cap = n;
+    }
+    // pageable sources: the copies are complete when the calls return
+    VX_HIP(hipMemcpyAsync(ctx->shade_items, h.data(), n * sizeof(ShadeArgs), hipMemcpyHostToDevice, S(stream)));
+    VX_HIP(hipMemcpyAsync(ctx->shade_block0, b0.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          S(stream)));
+    const int nacc = 256 * 6 + 6;
+    hipLaunchKernelGGL(k_zero_i64, dim3(cdiv(nacc, 256)), dim3(256), 0, S(stream), ctx->grad_acc, nacc);
+    hipLaunchKernelGGL(k_shade_bwd_batch, dim3((uint32_t)blocks), dim3(256), 0, S(stream), ctx->shade_items,
+                       ctx->shade_block0, n, ctx->grad_acc);
+    hipLaunchKernelGGL(k_shade_bwd_finish, dim3(cdiv(nacc, 256)), dim3(256), 0, S(stream), ctx->grad_acc, dpalette,
+                       dlight);
+    VX_LAUNCHED("k_shade_bwd_batch");
     return VX_OK;
 }
 

@@ -114,6 +114,8 @@ __global__ void k_zero_u32(uint32_t* p, int n);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
 __global__ void k_shade_fwd(ShadeArgs A);
 __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);
+__global__ void __launch_bounds__(256) k_shade_bwd_batch(const ShadeArgs* items, const uint32_t* block0, uint32_t n,
+                                                         long long* acc);
 __global__ void k_shade_bwd_finish(const long long* acc, float* dpal, float* dlight);
 __global__ void k_zero_i64(long long* p, int n);
 

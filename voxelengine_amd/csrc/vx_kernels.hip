@@ -1314,63 +1314,59 @@ VXD long long to_fixed(float v) { return __float2ll_rn(v * 1073741824.0f); }
 // Adjoint of k_shade_fwd with visibility fixed. Per-pixel partials are summed
 // into an LDS histogram of 256x6 (+6 light) int64 fixed-point cells, then
 // flushed with int64 atomics into the global accumulator acc[256*6+6].
-__global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc) {
-    __shared__ long long hist[256 * 6 + 6];
-    for (int j = threadIdx.x; j < 256 * 6 + 6; j += blockDim.x) hist[j] = 0;
-    __syncthreads();
-    float irr[3];
-    sky_irradiance(irr);
-    f3 ld = mk3(A.light[0], A.light[1], A.light[2]);
-    float ll = sqrtf(dot3(ld, ld));
-    f3 L = mk3(ld.x / ll, ld.y / ll, ld.z / ll);
-    float gl[6] = {0, 0, 0, 0, 0, 0};
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.npx; i += (size_t)gridDim.x * blockDim.x) {
-        uint32_t m = A.mat[i];
-        if (m == 0) continue;
-        ShadePix S;
-        shade_setup(A, i, L, S);
-        const float* e = A.palette + 6 * m;
-        float rough = e[3], metal = e[4];
-        float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
-        float F = f0d + (1.0f - f0d) * S.s5;
-        float kD = (1.0f - F) * (1.0f - metal);
-        float dF_dme = 0.96f * (1.0f - S.s5);
-        float dkD_dme = -dF_dme * (1.0f - metal) - (1.0f - F);
-        float gp[6] = {0, 0, 0, 0, 0, 0};
+// Partials of one pixel: palette entry m into hist (LDS, fixed point), light
+// into gl (per lane, summed per wave by the caller).
+VXD void shade_bwd_pixel(const ShadeArgs& A, size_t i, f3 L, float ll, const float irr[3], long long* hist,
+                         float gl[6]) {
+    uint32_t m = A.mat[i];
+    if (m == 0) return;
+    ShadePix S;
+    shade_setup(A, i, L, S);
+    const float* e = A.palette + 6 * m;
+    float rough = e[3], metal = e[4];
+    float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
+    float F = f0d + (1.0f - f0d) * S.s5;
+    float kD = (1.0f - F) * (1.0f - metal);
+    float dF_dme = 0.96f * (1.0f - S.s5);
+    float dkD_dme = -dF_dme * (1.0f - metal) - (1.0f - F);
+    float gp[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float gc = A.dl_drgb[3 * i + c];
-            float a = e[c];
-            float rad = A.light[3 + c] * S.sh;
-            float f0a = 0.04f * (1.0f - metal) + a * metal;
-            bool pick = (1.0f - rough) < f0a;
-            float mx = pick ? f0a : (1.0f - rough);
-            float fr = f0a + (mx - f0a) * S.s5;
-            float E = e[5] * 10.0f + (1.0f - fr) * S.A;
-            float dlo_da = ((kD / 3.14159265359f) * rad) * S.ndl;
-            float dmx_da = pick ? metal : 0.0f;
-            float dfr_da = metal + (dmx_da - metal) * S.s5;
-            gp[c] = gc * (dlo_da + (irr[c] * E - ((irr[c] * a) * S.A) * dfr_da));
-            float dfr_dro = pick ? 0.0f : -S.s5;
-            gp[3] += gc * (-((irr[c] * a) * S.A) * dfr_dro);
-            float dlo_dme = ((dkD_dme * a / 3.14159265359f) * rad) * S.ndl;
-            float df0a_dme = a - 0.04f;
-            float dmx_dme = pick ? df0a_dme : 0.0f;
-            float dfr_dme = df0a_dme + (dmx_dme - df0a_dme) * S.s5;
-            gp[4] += gc * (dlo_dme - ((irr[c] * a) * S.A) * dfr_dme);
-            gp[5] += gc * ((irr[c] * a) * 10.0f);
-            gl[3 + c] += gc * ((((kD * a) / 3.14159265359f) * S.sh) * S.ndl);
-            if (!(S.nl < 0.0f)) {
-                float k = gc * (((kD * a) / 3.14159265359f) * rad) / ll;
-                gl[0] += k * (S.N.x - L.x * S.nl);
-                gl[1] += k * (S.N.y - L.y * S.nl);
-                gl[2] += k * (S.N.z - L.z * S.nl);
-            }
+    for (int c = 0; c < 3; ++c) {
+        float gc = A.dl_drgb[3 * i + c];
+        float a = e[c];
+        float rad = A.light[3 + c] * S.sh;
+        float f0a = 0.04f * (1.0f - metal) + a * metal;
+        bool pick = (1.0f - rough) < f0a;
+        float mx = pick ? f0a : (1.0f - rough);
+        float fr = f0a + (mx - f0a) * S.s5;
+        float E = e[5] * 10.0f + (1.0f - fr) * S.A;
+        float dlo_da = ((kD / 3.14159265359f) * rad) * S.ndl;
+        float dmx_da = pick ? metal : 0.0f;
+        float dfr_da = metal + (dmx_da - metal) * S.s5;
+        gp[c] = gc * (dlo_da + (irr[c] * E - ((irr[c] * a) * S.A) * dfr_da));
+        float dfr_dro = pick ? 0.0f : -S.s5;
+        gp[3] += gc * (-((irr[c] * a) * S.A) * dfr_dro);
+        float dlo_dme = ((dkD_dme * a / 3.14159265359f) * rad) * S.ndl;
+        float df0a_dme = a - 0.04f;
+        float dmx_dme = pick ? df0a_dme : 0.0f;
+        float dfr_dme = df0a_dme + (dmx_dme - df0a_dme) * S.s5;
+        gp[4] += gc * (dlo_dme - ((irr[c] * a) * S.A) * dfr_dme);
+        gp[5] += gc * ((irr[c] * a) * 10.0f);
+        gl[3 + c] += gc * ((((kD * a) / 3.14159265359f) * S.sh) * S.ndl);
+        if (!(S.nl < 0.0f)) {
+            float k = gc * (((kD * a) / 3.14159265359f) * rad) / ll;
+            gl[0] += k * (S.N.x - L.x * S.nl);
+            gl[1] += k * (S.N.y - L.y * S.nl);
+            gl[2] += k * (S.N.z - L.z * S.nl);
         }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) atomicAdd((unsigned long long*)&hist[6 * m + k], (unsigned long long)to_fixed(gp[k]));
     }
-    // Light partials: per-lane fixed point, wave sum in int64, one LDS atomic per wave.
+#pragma unroll
+    for (int k = 0; k < 6; ++k) atomicAdd((unsigned long long*)&hist[6 * m + k], (unsigned long long)to_fixed(gp[k]));
+}
+
+// Light partials: per-lane fixed point, wave sum in int64, one LDS atomic per
+// wave; then the block's histogram into the global accumulator.
+VXD void shade_bwd_flush(long long* hist, const float gl[6], long long* acc) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         long long v = to_fixed(gl[k]);
@@ -1380,6 +1376,54 @@ __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc) 
     __syncthreads();
     for (int j = threadIdx.x; j < 256 * 6 + 6; j += blockDim.x)
         if (hist[j] != 0) atomicAdd((unsigned long long*)&acc[j], (unsigned long long)hist[j]);
+}
+
+VXD void shade_light(const float light[6], f3& L, float& ll) {
+    f3 ld = mk3(light[0], light[1], light[2]);
+    ll = sqrtf(dot3(ld, ld));
+    L = mk3(ld.x / ll, ld.y / ll, ld.z / ll);
+}
+
+__global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc) {
+    __shared__ long long hist[256 * 6 + 6];
+    for (int j = threadIdx.x; j < 256 * 6 + 6; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    float irr[3];
+    sky_irradiance(irr);
+    f3 L;
+    float ll;
+    shade_light(A.light, L, ll);
+    float gl[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < A.npx; i += (size_t)gridDim.x * blockDim.x)
+        shade_bwd_pixel(A, i, L, ll, irr, hist, gl);
+    shade_bwd_flush(hist, gl, acc);
+}
+
+// Batched adjoint: block b belongs to the item with block0[item] <= b; the
+// item's blocks stride over its pixels.
+__global__ void __launch_bounds__(256) k_shade_bwd_batch(const ShadeArgs* items, const uint32_t* block0, uint32_t n,
+                                                         long long* acc) {
+    __shared__ long long hist[256 * 6 + 6];
+    for (int j = threadIdx.x; j < 256 * 6 + 6; j += blockDim.x) hist[j] = 0;
+    __syncthreads();
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1u) >> 1;
+        if (block0[mid] <= blockIdx.x) lo = mid;
+        else hi = mid - 1u;
+    }
+    const ShadeArgs& A = items[lo];
+    const uint32_t nb = block0[lo + 1] - block0[lo];
+    float irr[3];
+    sky_irradiance(irr);
+    f3 L;
+    float ll;
+    shade_light(A.light, L, ll);
+    float gl[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = (size_t)(blockIdx.x - block0[lo]) * blockDim.x + threadIdx.x; i < A.npx;
+         i += (size_t)nb * blockDim.x)
+        shade_bwd_pixel(A, i, L, ll, irr, hist, gl);
+    shade_bwd_flush(hist, gl, acc);
 }
 
 __global__ void k_shade_bwd_finish(const long long* acc, float* dpal, float* dlight) {
