@@ -269,6 +269,43 @@ vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbu
                             const float light[6], const float* dl_drgb_dev, float* dpalette_dev,
                             float* dlight_dev, void* stream);
 
+/* ---------------------------------------------------------------- lights
+ * Point and spot lights: LightPointPipeline::Use + DrawLight
+ * (LightPointPipeline.h:58-100, LightPoint.frag:85-152) and LightSpotPipeline
+ * (LightSpotPipeline.h:60-104, LightSpot.frag:73-140). Each light's
+ * full-screen draw becomes one iteration of a per-pixel light loop. */
+enum { VX_LIGHT_POINT = 0, VX_LIGHT_SPOT = 1 };
+
+/* PointLight / SpotLight (Common.frag:15-30), world units. color is
+ * Color * Intensity and direction the light's WorldMatrix[2]
+ * (WorldRenderer.cpp:247,255). 64 bytes. */
+typedef struct {
+    int32_t type;
+    float position[3];
+    float range;
+    float color[3];
+    float attenuation;
+    float direction[3];    /* spot only */
+    float angle;           /* spot only */
+    float angle_attenuation;
+    int32_t pad;
+} vx_light;
+
+typedef struct {
+    int32_t point_march;   /* reference: VX_MARCH_SPARSE (LightPoint.frag:125) */
+    int32_t spot_march;    /* reference: VX_MARCH_SUPERSPARSE (LightSpot.frag:113) */
+    uint32_t seed;
+    int32_t frame;
+} vx_light_params;
+
+/* Adds every light's shadowed direct term (PBRDirectLight, PBR.frag:48-68)
+ * to rgb_dev (npx*3), lights in array order, as the reference's additive
+ * blending of the light draws does. lights_dev: device array of nlights
+ * (0..4096). fetches_dev (optional): the marchers' fetch counts per pixel. */
+vx_status vx_light_pass(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
+                        const float* palette_dev, const vx_light* lights_dev, uint32_t nlights,
+                        const vx_light_params* params, float* rgb_dev, uint32_t* fetches_dev, void* stream);
+
 /* One view of a batched adjoint: its G-buffer, fixed visibility, optional
  * SSAO and the loss gradient w.r.t. its shaded image (all device). */
 typedef struct {

@@ -56,6 +56,20 @@ class TraceParams(C.Structure):
                 ("jitter_shadow", C.c_int32), ("pad", C.c_int32)]
 
 
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", C.c_float * 3), ("range", C.c_float), ("color", C.c_float * 3),
+                ("attenuation", C.c_float), ("direction", C.c_float * 3), ("angle", C.c_float),
+                ("angle_attenuation", C.c_float), ("pad", C.c_int32)]
+
+
+class LightParams(C.Structure):
+    _fields_ = [("point_march", C.c_int32), ("spot_march", C.c_int32), ("seed", C.c_uint32), ("frame", C.c_int32)]
+
+
+LIGHT_DTYPE = np.dtype([("type", np.int32), ("position", np.float32, 3), ("range", np.float32),
+                        ("color", np.float32, 3), ("attenuation", np.float32), ("direction", np.float32, 3),
+                        ("angle", np.float32), ("angle_attenuation", np.float32), ("pad", np.int32)])
+
 _lib = None
 
 
@@ -106,6 +120,11 @@ def lib():
         L.vxo_gen_terrain.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_float, C.c_int32]
         L.vxo_gen_boxes.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_int32, C.c_int32,
                                     C.c_int32, C.c_int32]
+        L.vxo_log2.argtypes = [C.c_float]; L.vxo_log2.restype = C.c_float
+        L.vxo_exp2.argtypes = [C.c_float]; L.vxo_exp2.restype = C.c_float
+        L.vxo_pow.argtypes = [C.c_float, C.c_float]; L.vxo_pow.restype = C.c_float
+        L.vxo_light_pass.argtypes = [C.POINTER(Occ), C.POINTER(View), P, C.c_int32, C.POINTER(LightParams),
+                                     P, P, P, P, P, P, C.c_int32]
         _lib = L
     return _lib
 
@@ -335,3 +354,20 @@ def gen_boxes(vol, seed, nboxes, emin, emax, grounded=0):
     nz, ny, nx = vol.shape
     lib().vxo_gen_boxes(_p(vol), nx, ny, nz, seed & 0xFFFFFFFF, nboxes, emin, emax, grounded)
     return vol
+
+
+def light_params(point_march=1, spot_march=2, seed=0, frame=0) -> LightParams:
+    return LightParams(point_march, spot_march, seed & 0xFFFFFFFF, frame)
+
+
+def light_pass(vol, vd: dict, lights: np.ndarray, lp: LightParams, pos, nrm, mat, palette, rgb=None, nthreads=0):
+    """lights: LIGHT_DTYPE array. Adds every light's direct term to rgb (npx x 3,
+    zeros when None); returns (rgb, fetches)."""
+    npx = vd["width"] * vd["height"]
+    rgb = np.zeros((npx, 3), np.float32) if rgb is None else np.ascontiguousarray(rgb, np.float32).copy()
+    fetches = np.zeros(npx, np.uint32)
+    lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
+    O = occ(vol); v = view_struct(vd)
+    lib().vxo_light_pass(C.byref(O), C.byref(v), lights.ctypes.data, lights.size, C.byref(lp), _p(pos), _p(nrm),
+                         _p(mat), _p(np.ascontiguousarray(palette, np.float32)), _p(rgb), _p(fetches), nthreads)
+    return rgb, fetches

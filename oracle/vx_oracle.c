@@ -853,6 +853,158 @@ void vxo_shade_backward_f64(const vxo_view* vw, const float* pos, const float* n
 /* Terrain: occupied where y < H(x,z), H a 4-octave value-noise fBm with the */
 /* octave structure of Noise::GetOctave (Noise.cpp:93-107). Materials are a  */
 /* hash of the voxel position, uniform in 1..255, so box overlays commute.   */
+/* ---------------------------------------------------------------------------
+ * Point and spot lights.
+ * ------------------------------------------------------------------------ */
+static inline float bits_to_f(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static inline uint32_t f_to_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+/* log2 of a positive finite x: x = m * 2^e with m in [sqrt(1/2), sqrt(2)),
+ * ln(m) = z - z^2/2 + z^3 P(z), z = m - 1 (Cephes logf polynomial). */
+float vxo_log2(float x) {
+    int32_t e = 0;
+    if (x < 1.17549435e-38f) { x = x * 8388608.0f; e = -23; }   /* denormal: scale by 2^23 */
+    uint32_t b = f_to_bits(x);
+    e += (int32_t)((b >> 23) & 0xffu) - 127;
+    float m = bits_to_f((b & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    float z = m - 1.0f;
+    float z2 = z * z;
+    float p = 7.0376836292e-2f;
+    p = p * z - 1.1514610310e-1f;
+    p = p * z + 1.1676998740e-1f;
+    p = p * z - 1.2420140846e-1f;
+    p = p * z + 1.4249322787e-1f;
+    p = p * z - 1.6668057665e-1f;
+    p = p * z + 2.0000714765e-1f;
+    p = p * z - 2.4999993993e-1f;
+    p = p * z + 3.3333331174e-1f;
+    float ln = z + ((z * z2) * p - 0.5f * z2);
+    return ln * 1.44269504088896341f + (float)e;
+}
+
+/* 2^t: t = n + f, n = floor(t + 1/2), 2^f = 1 + f P(f) (Cephes exp2f
+ * polynomial), then two exact power-of-two scalings. */
+float vxo_exp2(float t) {
+    if (!(t == t)) return t;
+    if (t >= 128.0f) return INFINITY;
+    if (t < -150.0f) return 0.0f;
+    float n = floorf(t + 0.5f);
+    float f = t - n;
+    float p = 1.535336188319500e-4f;
+    p = p * f + 1.339887440266574e-3f;
+    p = p * f + 9.618437357674640e-3f;
+    p = p * f + 5.550332471162809e-2f;
+    p = p * f + 2.402264791363012e-1f;
+    p = p * f + 6.931472028550421e-1f;
+    float r = 1.0f + f * p;
+    int32_t k = (int32_t)n;
+    int32_t k1 = k / 2, k2 = k - k1;
+    r = r * bits_to_f((uint32_t)(k1 + 127) << 23);
+    return r * bits_to_f((uint32_t)(k2 + 127) << 23);
+}
+
+float vxo_pow(float x, float y) {
+    if (!(x == x) || !(y == y) || x < 0.0f) return NAN;
+    if (y == 0.0f || x == 1.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : INFINITY;
+    if (x == INFINITY) return y > 0.0f ? INFINITY : 0.0f;
+    return vxo_exp2(y * vxo_log2(x));
+}
+
+/* One pixel of the light passes: each light's full-screen draw
+ * (LightPointPipeline.h:58-100, LightSpotPipeline.h:60-104) blended additively
+ * in draw order, evaluated in world space. */
+static void light_pixel(const vxo_occ* v, const vxo_view* vw, const vxo_light* lights, int32_t nlights,
+                        const vxo_light_params* p, const float* pos, const float* nrm, const uint8_t* mat,
+                        const float* pal, float* rgb, uint32_t* fetches, int32_t i) {
+    int m = mat[i];
+    if (m == 0) return;
+    const float* pp = pos + 3 * (size_t)i;
+    const float* N = nrm + 3 * (size_t)i;
+    uint32_t pk = vxo_pixkey(p->seed, p->frame, (uint32_t)i);
+    float r0[4];
+    for (int k = 0; k < 4; ++k) r0[k] = vxo_rand(pk, (uint32_t)k);
+    float wpos[3], cam[3], vv[3], V[3];
+    for (int k = 0; k < 3; ++k) {
+        wpos[k] = pp[k] * 0.1f;                                  /* worldPos */
+        cam[k] = vw->cam_pos[k] * 10.0f;
+        vv[k] = cam[k] - pp[k];
+    }
+    normalize3(vv, V);
+    const float* e = pal + 6 * m;
+    float metal = e[4];
+    float ndv = g_max(dot3(N, V), 0.0f);
+    float x = g_max(1.0f - ndv, 0.0f);
+    float s5 = (((x * x) * x) * x) * x;
+    /* PBRDirectLight (PBR.frag:48-68): F0 = mix(0.04, 1, metallic) */
+    float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
+    float F = f0d + (1.0f - f0d) * s5;
+    float kD = (1.0f - F) * (1.0f - metal);
+    float rv[3];
+    vxo_cosine_hemisphere(r0[0], r0[1], rv);
+    for (int k = 0; k < 3; ++k) rv[k] = rv[k] * 0.1f;
+    rv[2] = rv[2] * vxo_sign(r0[2] - 0.5f);
+    uint32_t nf = 0;
+    for (int32_t l = 0; l < nlights; ++l) {
+        const vxo_light* L = &lights[l];
+        float ld[3];
+        for (int k = 0; k < 3; ++k) ld[k] = L->position[k] - wpos[k];      /* lightDir */
+        float dist = sqrtf(dot3(ld, ld));                                  /* lightDistance */
+        if (dist > L->range) continue;                                     /* discard */
+        const int spot = L->type == VXO_LIGHT_SPOT;
+        float wd[3], hit_dist;
+        if (spot) {
+            float nl[3];
+            normalize3(ld, nl);
+            for (int k = 0; k < 3; ++k) wd[k] = nl[k] * 10.0f;
+            hit_dist = dist * 10.0f;
+        } else {
+            for (int k = 0; k < 3; ++k) wd[k] = ld[k];
+            hit_dist = dist * 10.5f;
+        }
+        float mx[3], wdn[3], o[3];
+        for (int k = 0; k < 3; ++k) mx[k] = wd[k] * (1.0f - 0.5f) + rv[k] * 0.5f;
+        normalize3(mx, wdn);
+        for (int k = 0; k < 3; ++k) {
+            float wcp = wpos[k] * 10.0f;
+            wcp = wcp + wdn[k] * (r0[3] * 1.0f);
+            wcp = wcp + rv[k] * 2.5f;
+            o[k] = wcp + N[k] * 0.5f;
+        }
+        int32_t hit;
+        float d = march_dist(v, spot ? p->spot_march : p->point_march, o, wdn, hit_dist, &nf, &hit);
+        float shadow = (d < hit_dist) ? 0.0f : 1.0f;
+        float att = g_clamp(L->range - dist, 0.0f, 1.0f) / vxo_pow(dist, L->attenuation);
+        if (spot) {
+            float nl[3];
+            normalize3(ld, nl);
+            float ad = g_max(dot3(nl, L->direction) - (1.0f - L->angle), 0.0f) / L->angle;
+            att = (vxo_pow(ad, L->angle_attenuation) * g_clamp(L->range - dist, 0.0f, 1.0f)) /
+                  vxo_pow(dist, L->attenuation);
+        }
+        float ndl = g_max(dot3(N, ld), 0.0f);
+        for (int c = 0; c < 3; ++c) {
+            float rad = (L->color[c] * att) * shadow;
+            float lo = (((kD * e[c]) / PI_F) * rad) * ndl;
+            rgb[3 * (size_t)i + c] = rgb[3 * (size_t)i + c] + lo;
+        }
+    }
+    if (fetches) fetches[i] = nf;
+}
+
+void vxo_light_pass(const vxo_occ* v, const vxo_view* vw, const vxo_light* lights, int32_t nlights,
+                    const vxo_light_params* p, const float* pos, const float* nrm, const uint8_t* mat,
+                    const float* palette, float* rgb, uint32_t* fetches, int32_t nthreads) {
+    const int32_t npx = vw->width * vw->height;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+    for (int32_t i = 0; i < npx; ++i) light_pixel(v, vw, lights, nlights, p, pos, nrm, mat, palette, rgb, fetches, i);
+    (void)nthreads;
+}
+
 /* ------------------------------------------------------------------------ */
 static inline uint32_t vox_material(uint32_t seed, int32_t x, int32_t y, int32_t z) {
     uint32_t h = vxo_hash(seed ^ 0x68bc21ebu);

@@ -143,6 +143,42 @@ void vxo_shade_backward_f64(const vxo_view* view, const float* pos, const float*
                             const float* ssao, const double* palette, const double light[6],
                             const double* dl_drgb, double* dpalette, double dlight[6]);
 
+/* ---- point / spot lights (LightPoint.frag:85-152, LightSpot.frag:73-140) ---- */
+/* pow(x, y) of the attenuation terms: exp2(y * log2(x)) from the build's own
+ * log2 / exp2 (range reduction + Cephes single-precision polynomials, only
+ * + - * /), replacing the driver's pow. Defined for x >= 0. */
+float vxo_log2(float x);
+float vxo_exp2(float t);
+float vxo_pow(float x, float y);
+
+enum { VXO_LIGHT_POINT = 0, VXO_LIGHT_SPOT = 1 };
+
+/* PointLight / SpotLight (Common.frag:15-30); direction = the light's
+ * WorldMatrix[2] (WorldRenderer.cpp:255), color = Color * Intensity. */
+typedef struct {
+    int32_t type;
+    float position[3];
+    float range;
+    float color[3];
+    float attenuation;
+    float direction[3];
+    float angle;
+    float angle_attenuation;
+    int32_t pad;
+} vxo_light;
+
+typedef struct {
+    int32_t point_march;    /* reference: SPARSE (LightPoint.frag:125) */
+    int32_t spot_march;     /* reference: SUPERSPARSE (LightSpot.frag:113) */
+    uint32_t seed;
+    int32_t frame;
+} vxo_light_params;
+
+/* Adds each light's direct term to rgb (npx*3), lights in array order. */
+void vxo_light_pass(const vxo_occ* v, const vxo_view* view, const vxo_light* lights, int32_t nlights,
+                    const vxo_light_params* p, const float* pos, const float* nrm, const uint8_t* mat,
+                    const float* palette, float* rgb, uint32_t* fetches, int32_t nthreads);
+
 /* ---- synthetic scenes (inputs, not reference code) ---- */
 void vxo_gen_terrain(uint8_t* vol, int32_t nx, int32_t ny, int32_t nz, uint32_t seed,
                      float fill, int32_t nthreads);

@@ -22,6 +22,7 @@ VX_ERR_STATE = -6
 
 MARCH_DDA, MARCH_SPARSE, MARCH_SUPERSPARSE = 0, 1, 2
 INST_UPDATE, INST_DESTROY = 1, 2
+LIGHT_POINT, LIGHT_SPOT = 0, 1
 
 P = C.c_void_p
 f16 = C.c_float * 16
@@ -74,6 +75,16 @@ class BatchItem(C.Structure):
     _fields_ = [("volume", P), ("view", View), ("gbuf", GBuf), ("out", TraceOut)]
 
 
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", C.c_float * 3), ("range", C.c_float), ("color", C.c_float * 3),
+                ("attenuation", C.c_float), ("direction", C.c_float * 3), ("angle", C.c_float),
+                ("angle_attenuation", C.c_float), ("pad", C.c_int32)]
+
+
+class LightParams(C.Structure):
+    _fields_ = [("point_march", C.c_int32), ("spot_march", C.c_int32), ("seed", C.c_uint32), ("frame", C.c_int32)]
+
+
 class ShadeItem(C.Structure):
     _fields_ = [("view", View), ("gbuf", GBuf), ("vis", TraceOut), ("ssao", P), ("dl_drgb", P)]
 
@@ -123,6 +134,8 @@ SIGNATURES = {
                           P, P], C.c_int32),
     "vx_shade_backward": ([P, C.POINTER(View), C.POINTER(GBuf), C.POINTER(TraceOut), P, P,
                            C.POINTER(C.c_float), P, P, P, P], C.c_int32),
+    "vx_light_pass": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, P, C.c_uint32, C.POINTER(LightParams), P, P, P],
+                      C.c_int32),
     "vx_shade_backward_batch": ([P, C.POINTER(ShadeItem), C.c_uint32, P, C.POINTER(C.c_float), P, P, P],
                                 C.c_int32),
 }

@@ -330,3 +330,24 @@ def shade_backward_batch(ctx, items, palette, light, dpalette, dlight, stream=No
         arr[k] = L.ShadeItem(view_struct(vd), gbuf.struct(), vis.struct(), _ptr(ssao), _ptr(dl))
     check(L.lib().vx_shade_backward_batch(ctx.h, arr, len(items), _ptr(palette), _light(light), _ptr(dpalette),
                                           _ptr(dlight), _stream(stream)))
+
+
+LIGHT_DTYPE = np.dtype([("type", np.int32), ("position", np.float32, 3), ("range", np.float32),
+                        ("color", np.float32, 3), ("attenuation", np.float32), ("direction", np.float32, 3),
+                        ("angle", np.float32), ("angle_attenuation", np.float32), ("pad", np.int32)])
+
+
+def lights_to_device(lights: np.ndarray) -> torch.Tensor:
+    """LIGHT_DTYPE array -> device buffer of vx_light (64 B each)."""
+    lights = np.ascontiguousarray(lights, LIGHT_DTYPE)
+    return torch.from_numpy(lights.view(np.uint8).copy()).cuda()
+
+
+def light_pass(ctx, vol, vd, gbuf, palette, lights_dev, nlights, rgb, point_march=L.MARCH_SPARSE,
+               spot_march=L.MARCH_SUPERSPARSE, seed=0, frame=0, fetches=None, stream=None):
+    """Adds the point/spot lights' shadowed direct light to rgb (npx x 3 fp32)."""
+    v = view_struct(vd); g = gbuf.struct()
+    p = L.LightParams(point_march, spot_march, seed & 0xFFFFFFFF, frame)
+    check(L.lib().vx_light_pass(ctx.h, vol.h, C.byref(v), C.byref(g), _ptr(palette),
+                                _ptr(lights_dev) if nlights else None, nlights, C.byref(p), _ptr(rgb),
+                                _ptr(fetches), _stream(stream)))

@@ -992,6 +992,32 @@ extern "C" vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const floa
     return VX_OK;
 }
 
+extern "C" vx_status vx_light_pass(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                                   const float* palette, const vx_light* lights, uint32_t nlights,
+                                   const vx_light_params* params, float* rgb, uint32_t* fetches, void* stream) {
+    VX_REQUIRE(ctx && vol && g && palette && params && rgb, "ctx/volume/gbuf/palette/params/rgb is NULL");
+    VX_REQUIRE(nlights <= 4096 && (nlights == 0 || lights), "lights: 0..4096 with a device array");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g->pos && g->normal && g->material, "gbuf has a NULL pointer");
+    auto okm = [](int32_t m) { return m == VX_MARCH_DDA || m == VX_MARCH_SPARSE || m == VX_MARCH_SUPERSPARSE; };
+    VX_REQUIRE(okm(params->point_march) && okm(params->spot_march), "unknown marcher");
+    if (vx_status st = use_device(ctx)) return st;
+    LightArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.view = *view;
+    A.tiles = tile_map(view->width, view->height);
+    A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
+    A.palette = palette;
+    A.lights = lights;
+    A.nlights = nlights;
+    A.params = *params;
+    A.rgb = rgb;
+    A.fetches = fetches;
+    hipLaunchKernelGGL(k_lights, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
+    VX_LAUNCHED("k_lights");
+    return VX_OK;
+}
+
 static vx_status shade_args(const vx_view* view, const vx_gbuf* g, const vx_trace_out* vis, const float* ssao,
                             const float* palette, const float light[6], ShadeArgs& A) {
     if (vx_status st = check_view(view)) return st;

@@ -69,6 +69,57 @@ VXD void sincos_vx(float x, float* s, float* c) {
     }
 }
 
+// log2 / exp2 / pow for the light attenuation terms (the driver's pow is
+// replaced): range reduction plus the Cephes single-precision log and exp2
+// polynomials, only + - * /, the same expressions as the oracle's.
+VXD float log2_vx(float x) {
+    int32_t e = 0;
+    if (x < 1.17549435e-38f) { x = x * 8388608.0f; e = -23; }
+    uint32_t b = __float_as_uint(x);
+    e += (int32_t)((b >> 23) & 0xffu) - 127;
+    float m = __uint_as_float((b & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    float z = m - 1.0f;
+    float z2 = z * z;
+    float p = 7.0376836292e-2f;
+    p = p * z - 1.1514610310e-1f;
+    p = p * z + 1.1676998740e-1f;
+    p = p * z - 1.2420140846e-1f;
+    p = p * z + 1.4249322787e-1f;
+    p = p * z - 1.6668057665e-1f;
+    p = p * z + 2.0000714765e-1f;
+    p = p * z - 2.4999993993e-1f;
+    p = p * z + 3.3333331174e-1f;
+    float ln = z + ((z * z2) * p - 0.5f * z2);
+    return ln * 1.44269504088896341f + (float)e;
+}
+VXD float exp2_vx(float t) {
+    if (!(t == t)) return t;
+    if (t >= 128.0f) return __uint_as_float(0x7f800000u);
+    if (t < -150.0f) return 0.0f;
+    float n = floorf(t + 0.5f);
+    float f = t - n;
+    float p = 1.535336188319500e-4f;
+    p = p * f + 1.339887440266574e-3f;
+    p = p * f + 9.618437357674640e-3f;
+    p = p * f + 5.550332471162809e-2f;
+    p = p * f + 2.402264791363012e-1f;
+    p = p * f + 6.931472028550421e-1f;
+    float r = 1.0f + f * p;
+    int32_t k = (int32_t)n;
+    int32_t k1 = k / 2, k2 = k - k1;
+    r = r * __uint_as_float((uint32_t)(k1 + 127) << 23);
+    return r * __uint_as_float((uint32_t)(k2 + 127) << 23);
+}
+VXD float pow_vx(float x, float y) {
+    const float inf = __uint_as_float(0x7f800000u);
+    if (!(x == x) || !(y == y) || x < 0.0f) return __uint_as_float(0x7fc00000u);
+    if (y == 0.0f || x == 1.0f) return 1.0f;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : inf;
+    if (x == inf) return y > 0.0f ? inf : 0.0f;
+    return exp2_vx(y * log2_vx(x));
+}
+
 // Counter-hash RNG (lowbias32) replacing the blue-noise texture.
 VXD uint32_t hash32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352du;

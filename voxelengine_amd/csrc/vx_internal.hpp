@@ -67,6 +67,20 @@ struct MarchArgs {
     float maxt[3];             // shadow, AO, spec
 };
 
+struct LightArgs {
+    vx_view view;
+    TileMap tiles;
+    const float* pos;
+    const float* nrm;
+    const uint8_t* mat;
+    const float* palette;
+    const vx_light* lights;
+    uint32_t nlights;
+    vx_light_params params;
+    float* rgb;
+    uint32_t* fetches;
+};
+
 struct PrimaryArgs {
     vx_view view;
     TileMap tiles;
@@ -111,6 +125,7 @@ __global__ void k_level_up(uint64_t* words, const uint64_t* lower, int32_t lwx, 
 __global__ void k_trace_rays(OccView V, const float* rays, uint32_t n, int32_t march, vx_ray_result* out);
 __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A);
 __global__ void k_zero_u32(uint32_t* p, int n);
+__global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
 __global__ void k_shade_fwd(ShadeArgs A);
 __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);

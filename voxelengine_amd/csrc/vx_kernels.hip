@@ -1118,6 +1118,89 @@ __global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32
 }
 
 // ===========================================================================
+// Point and spot lights (LightPoint.frag:85-152, LightSpot.frag:73-140): one
+// thread per pixel, the lights' draws as a loop in draw order, world space.
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
+    int32_t px, py;
+    if (!tile_pixel(A.tiles, px, py)) return;
+    const size_t i = (size_t)py * A.view.width + px;
+    const uint32_t m = A.mat[i];
+    if (m == 0) {
+        if (A.fetches) A.fetches[i] = 0;
+        return;
+    }
+    const f3 pp = load3(A.pos + 3 * i), N = load3(A.nrm + 3 * i);
+    const uint32_t pk = pixkey(A.params.seed, A.params.frame, (uint32_t)i);
+    float r0[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r0[k] = rand01(pk, (uint32_t)k);
+    const f3 wpos = mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f);
+    const f3 cam = mk3(A.view.cam_pos[0] * 10.0f, A.view.cam_pos[1] * 10.0f, A.view.cam_pos[2] * 10.0f);
+    const f3 Vd = normalize3(mk3(cam.x - pp.x, cam.y - pp.y, cam.z - pp.z));
+    const float* e = A.palette + 6 * m;
+    const float metal = e[4];
+    const float ndv = gmax(dot3(N, Vd), 0.0f);
+    const float x = gmax(1.0f - ndv, 0.0f);
+    const float s5 = (((x * x) * x) * x) * x;
+    // PBRDirectLight (PBR.frag:48-68): F0 = mix(0.04, 1, metallic)
+    const float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
+    const float F = f0d + (1.0f - f0d) * s5;
+    const float kD = (1.0f - F) * (1.0f - metal);
+    f3 rv = cosine_hemisphere(r0[0], r0[1]);
+    rv = mk3(rv.x * 0.1f, rv.y * 0.1f, rv.z * 0.1f);
+    rv.z = rv.z * gsign(r0[2] - 0.5f);
+    float acc[3] = {A.rgb[3 * i + 0], A.rgb[3 * i + 1], A.rgb[3 * i + 2]};
+    uint32_t nf = 0;
+    for (uint32_t l = 0; l < A.nlights; ++l) {
+        const vx_light& L = A.lights[l];
+        const f3 ld = mk3(L.position[0] - wpos.x, L.position[1] - wpos.y, L.position[2] - wpos.z);
+        const float dist = sqrtf(dot3(ld, ld));
+        if (dist > L.range) continue;
+        const bool spot = L.type == VX_LIGHT_SPOT;
+        f3 wd;
+        float hit_dist;
+        if (spot) {
+            const f3 nl = normalize3(ld);
+            wd = mk3(nl.x * 10.0f, nl.y * 10.0f, nl.z * 10.0f);
+            hit_dist = dist * 10.0f;
+        } else {
+            wd = ld;
+            hit_dist = dist * 10.5f;
+        }
+        const f3 wdn = normalize3(mk3(wd.x * (1.0f - 0.5f) + rv.x * 0.5f, wd.y * (1.0f - 0.5f) + rv.y * 0.5f,
+                                      wd.z * (1.0f - 0.5f) + rv.z * 0.5f));
+        f3 o;
+        {
+            float c0 = wpos.x * 10.0f, c1 = wpos.y * 10.0f, c2 = wpos.z * 10.0f;
+            c0 = c0 + wdn.x * (r0[3] * 1.0f); c1 = c1 + wdn.y * (r0[3] * 1.0f); c2 = c2 + wdn.z * (r0[3] * 1.0f);
+            c0 = c0 + rv.x * 2.5f; c1 = c1 + rv.y * 2.5f; c2 = c2 + rv.z * 2.5f;
+            o = mk3(c0 + N.x * 0.5f, c1 + N.y * 0.5f, c2 + N.z * 0.5f);
+        }
+        int32_t hit;
+        const float d = march_dist(V, spot ? A.params.spot_march : A.params.point_march, o, wdn, hit_dist, nf, hit);
+        const float shadow = (d < hit_dist) ? 0.0f : 1.0f;
+        float att = gclamp(L.range - dist, 0.0f, 1.0f) / pow_vx(dist, L.attenuation);
+        if (spot) {
+            const f3 nl = normalize3(ld);
+            const f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
+            const float ad = gmax(dot3(nl, dir) - (1.0f - L.angle), 0.0f) / L.angle;
+            att = (pow_vx(ad, L.angle_attenuation) * gclamp(L.range - dist, 0.0f, 1.0f)) / pow_vx(dist, L.attenuation);
+        }
+        const float ndl = gmax(dot3(N, ld), 0.0f);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float rad = (L.color[c] * att) * shadow;
+            acc[c] = acc[c] + (((kD * e[c]) / 3.14159265359f) * rad) * ndl;
+        }
+    }
+    A.rgb[3 * i + 0] = acc[0];
+    A.rgb[3 * i + 1] = acc[1];
+    A.rgb[3 * i + 2] = acc[2];
+    if (A.fetches) A.fetches[i] = nf;
+}
+
+// ===========================================================================
 // Primary hit: GeometryVoxel.frag (LOD 1, MIP_COUNT 2, TRANSPARENCY 1) for the
 // world model with an identity WorldMatrix.
 // ===========================================================================
