@@ -2,7 +2,8 @@
 // system and render-pass shapes, so engine code written against VoxelEngine's
 // ShadowVoxSystem (Sources/World/Systems/ShadowVoxSystem.h:9-37) and
 // Light*Pipeline::Use (Sources/Graphics/Pipelines/LightAmbientPipeline.h:35,
-// LightReflectionPipeline.h:34) drops onto the gfx950 path.
+// LightReflectionPipeline.h:34, LightPointPipeline.h:58-100,
+// LightSpotPipeline.h:60-104) drops onto the gfx950 path.
 //
 // Header-only; link with libvx_amd.so. Errors throw vx::Error, as the
 // reference's CHECK throws (Sources/Core/Core.h:83-86).
@@ -16,6 +17,8 @@
 #include <string>
 #include <utility>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "vx_api.h"
 
@@ -225,6 +228,92 @@ public:
         p.n_spec = 1;
         check(vx_trace(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, &p, &reflectionOut, cmd));
     }
+};
+
+// LightPointPipeline / LightSpotPipeline (LightPointPipeline.h:58-100,
+// LightSpotPipeline.h:60-104): Use() opens the pass, the callback calls
+// DrawLight once per light (at most 64 per type, as the reference's buffers),
+// and the pass ends with one light-loop launch that adds every light's
+// shadowed direct term to the light buffer. lightsDevice is caller-owned
+// device memory for 64 vx_light entries; lightBufferRGB holds npx*3 floats.
+class LightPointPipeline {
+public:
+    static constexpr int kMaxLights = 64;
+    static LightPointPipeline& Get() {
+        static LightPointPipeline p;
+        return p;
+    }
+    template <class F>
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB, vx_volume* shadowVox,
+             const float* palette, const vx_light_params& params, vx_light* lightsDevice, float* lightBufferRGB,
+             F&& callback) {
+        lights_.clear();
+        callback(*this);
+        if (lights_.empty()) return;
+        check_hip(hipMemcpyAsync(lightsDevice, lights_.data(), lights_.size() * sizeof(vx_light),
+                                 hipMemcpyHostToDevice, static_cast<hipStream_t>(cmd)));
+        check(vx_light_pass(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, lightsDevice,
+                            (uint32_t)lights_.size(), &params, lightBufferRGB, nullptr, cmd));
+    }
+    // DrawLight(position, range, color * intensity, attenuation)
+    void DrawLight(const float position[3], float range, const float color[3], float attenuation) {
+        if ((int)lights_.size() >= kMaxLights) return;   // the reference drops lights past its buffer
+        vx_light l;
+        std::memset(&l, 0, sizeof(l));
+        l.type = VX_LIGHT_POINT;
+        std::memcpy(l.position, position, sizeof(l.position));
+        l.range = range;
+        std::memcpy(l.color, color, sizeof(l.color));
+        l.attenuation = attenuation;
+        lights_.push_back(l);
+    }
+
+private:
+    static void check_hip(hipError_t e) {
+        if (e != hipSuccess) throw Error(VX_ERR_DEVICE, hipGetErrorString(e));
+    }
+    std::vector<vx_light> lights_;
+};
+
+class LightSpotPipeline {
+public:
+    static constexpr int kMaxLights = 64;
+    static LightSpotPipeline& Get() {
+        static LightSpotPipeline p;
+        return p;
+    }
+    template <class F>
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB, vx_volume* shadowVox,
+             const float* palette, const vx_light_params& params, vx_light* lightsDevice, float* lightBufferRGB,
+             F&& callback) {
+        lights_.clear();
+        callback(*this);
+        if (lights_.empty()) return;
+        hipError_t e = hipMemcpyAsync(lightsDevice, lights_.data(), lights_.size() * sizeof(vx_light),
+                                      hipMemcpyHostToDevice, static_cast<hipStream_t>(cmd));
+        if (e != hipSuccess) throw Error(VX_ERR_DEVICE, hipGetErrorString(e));
+        check(vx_light_pass(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, lightsDevice,
+                            (uint32_t)lights_.size(), &params, lightBufferRGB, nullptr, cmd));
+    }
+    // DrawLight(position, range, color * intensity, attenuation, WorldMatrix[2], angle, angleAttenuation)
+    void DrawLight(const float position[3], float range, const float color[3], float attenuation,
+                   const float direction[3], float angle, float angleAttenuation) {
+        if ((int)lights_.size() >= kMaxLights) return;
+        vx_light l;
+        std::memset(&l, 0, sizeof(l));
+        l.type = VX_LIGHT_SPOT;
+        std::memcpy(l.position, position, sizeof(l.position));
+        l.range = range;
+        std::memcpy(l.color, color, sizeof(l.color));
+        l.attenuation = attenuation;
+        std::memcpy(l.direction, direction, sizeof(l.direction));
+        l.angle = angle;
+        l.angle_attenuation = angleAttenuation;
+        lights_.push_back(l);
+    }
+
+private:
+    std::vector<vx_light> lights_;
 };
 
 }  // namespace vx
