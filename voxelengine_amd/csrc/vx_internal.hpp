@@ -60,7 +60,7 @@ struct MarchArgs {
     uint32_t* counters;        // band counters, listed-ray count (kCounterStride apart)
     uint32_t* deferred;        // slots of rays left to the literal path
     uint32_t deferred_cap;     // capacity of deferred (every slot fits)
-    float* res;                // per-ray value, slot = ((tile * nrays + r) << 6) | pixel-in-tile
+    float* res;                // per-ray best_t of a hit, a marker NaN otherwise; slot = ((tile * nrays + r) << 6) | pixel-in-tile
     uint16_t* fres;            // per-ray fetch count, or null
     int32_t nrays;
     int32_t refill;            // not-running lanes that trigger a refill

@@ -557,9 +557,10 @@ VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
 // band and generates their rays 64 at a time into a ring in LDS: one ray index
 // over the tile's valid pixels per batch, so the set-up code runs with one ray
 // class and all lanes. A lane whose ray ended takes the next ray from the
-// ring. Every ray owns one result slot (its occlusion value and fetch count),
+// ring. Every ray owns one result slot (best_t of a hit, and the fetch count),
 // slot = ((tile * nrays + r) << 6) | pixel-in-tile; k_resolve folds a pixel's
-// slots in ray order, so the float sums keep the reference order.
+// slots in ray order, so the float sums keep the reference order. A slot holds
+// best_t of a hit and a marker NaN otherwise; k_resolve derives the values.
 //
 // Exact fast path. For a ray with every |dir| in [2^-100, 3e38] and |origin| <
 // 2^22, t_max and t_delta stay finite and t_max >= 0 for the whole march, so:
@@ -580,7 +581,7 @@ VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
 // ===========================================================================
 
 constexpr int kRing = 128;       // ring entries per wave
-constexpr int kRingFields = 10;  // t_max[3], t_delta[3], x|y<<16, z|signs<<16|kind<<19, slot, item
+constexpr int kRingFields = 11;  // t_max[3], t_delta[3], x|y<<16, z|signs<<16|kind<<19, slot, item, grid word
 
 VXD bool fast_ray_ok(f3 o, f3 d) {
     const float lo = 7.8886091e-31f;   // 2^-100
@@ -687,7 +688,8 @@ VXD int32_t pixel_ray(const TraceArgs& A, const PixState& S, f3 wd, int32_t r, f
 }
 
 // Value a pixel folds for one ray (shadow: lit, AO and spec: min(dist, maxt) /
-// maxt with dist = maxt unless hit) and the ray's fetch count.
+// maxt with dist = maxt unless hit) and the ray's fetch count. The march
+// stores per ray only best_t of a hit (kNoHitBits otherwise); k_resolve folds.
 VXD float ray_value(int32_t kind, int32_t term, float t, float maxt) {
     bool hit = term == 1;
     if (kind == RAY_SHADOW) return hit ? 0.0f : 1.0f;
@@ -697,6 +699,11 @@ VXD float ray_value(int32_t kind, int32_t term, float t, float maxt) {
 VXD uint32_t ray_fetches(int32_t term, int32_t nt) {
     return (uint32_t)(term == 0 ? 4 * nt : nt + (term == 1 ? 1 : 0));
 }
+
+// Slot value of a ray that did not hit: a NaN whose payload the path's
+// arithmetic does not produce (it yields the default NaN), so every hit's
+// best_t, including -inf or NaN from the literal path, stays distinguishable.
+constexpr uint32_t kNoHitBits = 0x7fc0beefu;
 
 // best_t of the current voxel from t_max (see the fast-path identities).
 VXD float best_t_of(float tm0, float tm1, float tm2) {
@@ -742,7 +749,7 @@ VXD void ray_origin_voxel(f3 o, int32_t cv[3]) {
 
 // Writes the ring entry of a fast-path ray that starts inside the bounds.
 VXD void ring_put_ray(uint32_t (*ring)[kRing], uint32_t e, f3 o, f3 d, const int32_t cv[3], int32_t kind,
-                      uint32_t slot, uint32_t item) {
+                      uint32_t slot, uint32_t item, uint32_t gx, uint32_t gxy) {
     float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
     uint32_t signs = 0;
 #pragma unroll
@@ -758,6 +765,8 @@ VXD void ring_put_ray(uint32_t (*ring)[kRing], uint32_t e, f3 o, f3 d, const int
     ring[7][e] = (uint32_t)(cv[2] + 4) | (signs << 16) | ((uint32_t)kind << 19);
     ring[8][e] = slot;
     ring[9][e] = item;
+    ring[10][e] = __umul24((uint32_t)((cv[2] + 4) >> 2), gxy) +
+                  (__umul24((uint32_t)((cv[1] + 4) >> 2), gx) + (uint32_t)((cv[0] + 4) >> 2));
 }
 
 // The ray a lane marches (coordinates as below); meta = n << 20 | kind << 16 |
@@ -829,15 +838,15 @@ k_march(MarchArgs A) {
                 const uint32_t kind = (R.meta >> 16) & 3u, n = R.meta >> 20;
                 const MarchItem& I = march_item<SINGLE>(A, R.meta & 0xffffu);
                 int32_t term = 0;
-                float t = 0.0f;
+                float v = __uint_as_float(kNoHitBits);
                 if (stopped) {
                     bool oob = (uint32_t)(R.x - 4) > (uint32_t)I.lim[0] ||
                                (uint32_t)((R.y >> 2) - 4) > (uint32_t)I.lim[1] ||
                                (uint32_t)((R.z >> 4) - 4) > (uint32_t)I.lim[2];
                     term = oob ? 2 : 1;
-                    t = best_t_of(R.tm0, R.tm1, R.tm2);
+                    if (!oob) v = best_t_of(R.tm0, R.tm1, R.tm2);
                 }
-                A.res[R.slot] = ray_value((int32_t)kind, term, t, R.maxt);
+                A.res[R.slot] = v;
                 if (A.fres) A.fres[R.slot] = (uint16_t)ray_fetches(term, (int32_t)n);
                 hasray = false;
                 stopped = false;
@@ -887,14 +896,15 @@ k_march(MarchArgs A) {
                             push = true;
                         } else {
                             // starts outside: term 2 at step 0
-                            A.res[slot] = ray_value(kind, 2, 0.0f, maxt);
+                            A.res[slot] = __uint_as_float(kNoHitBits);
                             if (A.fres) A.fres[slot] = 0;
                         }
                     }
                 }
                 const uint64_t pm = __ballot(push);
-                if (push) ring_put_ray(ring, (tail + (uint32_t)__popcll(pm & lane_lt)) & (kRing - 1), o, d, cv, kind,
-                                       slot, gitem);
+                if (push)
+                    ring_put_ray(ring, (tail + (uint32_t)__popcll(pm & lane_lt)) & (kRing - 1), o, d, cv, kind, slot,
+                                 gitem, (uint32_t)I.gx, (uint32_t)I.gxy);
                 tail += (uint32_t)__popcll(pm);
                 gr++;
             }
@@ -929,7 +939,7 @@ k_march(MarchArgs A) {
                         ggx = (uint32_t)I.gx;
                         ggxy = (uint32_t)I.gxy;
                     }
-                    R.ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
+                    R.ci = ring[10][e];
                     R.cw = grid[R.ci];
                     hasray = true;
                     running = true;
@@ -988,10 +998,10 @@ __global__ void __launch_bounds__(256) k_march_rest(MarchArgs A) {
         pixel_setup(I.T, i, P, wd);
         f3 o, d;
         float maxt;
-        int32_t kind = pixel_ray(I.T, P, wd, (int32_t)r, o, d, maxt);
+        (void)pixel_ray(I.T, P, wd, (int32_t)r, o, d, maxt);
         RayOut R;
         dda_exact(I.V, o, d, maxt, R);
-        A.res[slot] = ray_value(kind, R.term, R.t, maxt);
+        A.res[slot] = R.hit ? R.t : __uint_as_float(kNoHitBits);
         if (A.fres) A.fres[slot] = (uint16_t)R.fetches;
     }
 }
@@ -1018,10 +1028,16 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
     const size_t base = (((size_t)gt * (size_t)A.nrays) << 6) | l;
     for (int32_t r = 0; r < A.nrays; ++r) {
         const size_t s = base + ((size_t)r << 6);
-        float v = A.res[s];
-        if (r < P.n_shadow) lit = v != 0.0f ? 1u : 0u;
-        else if (r < P.n_shadow + P.n_ao) aosum = aosum + v * v;
-        else spv = v;
+        const float v = A.res[s];
+        const bool hit = __float_as_uint(v) != kNoHitBits;
+        if (r < P.n_shadow) {
+            lit = hit ? 0u : 1u;
+        } else {
+            const float maxt = r < P.n_shadow + P.n_ao ? P.ao_maxt : P.spec_maxt;
+            const float q = ray_value(RAY_AO, hit ? 1 : 0, v, maxt);
+            if (r < P.n_shadow + P.n_ao) aosum = aosum + q * q;
+            else spv = q;
+        }
         if (A.fres) nf += A.fres[s];
     }
     T.shadow[i] = (uint8_t)lit;
