@@ -51,6 +51,10 @@ private:
 class VoxAsset {
 public:
     VoxAsset(Context& ctx, int sx, int sy, int sz) { check(vx_model_create(ctx.get(), sx, sy, sz, &h_)); }
+    // A .v file (VoxAsset::Serialize + OnLoad, VoxAsset.h:36-50).
+    VoxAsset(Context& ctx, const std::string& path, void* stream = nullptr) {
+        check(vx_model_load(ctx.get(), path.c_str(), &h_, stream));
+    }
     ~VoxAsset() { vx_model_destroy(h_); }
     VoxAsset(const VoxAsset&) = delete;
     VoxAsset& operator=(const VoxAsset&) = delete;
@@ -79,6 +83,60 @@ private:
     vx_model* h_ = nullptr;
 };
 
+// PalleteAsset (Sources/Asset/PalleteAsset.h:27-70) as the 256 x 6 fp32
+// palette the passes read (host memory; copy it to the device to use it).
+inline std::vector<float> LoadPallete(const std::string& path) {
+    std::vector<float> pal(256 * 6);
+    check(vx_palette_load(path.c_str(), pal.data()));
+    return pal;
+}
+
+// PrefabAsset (Sources/Asset/PrefabAsset.h:14-40): the entities
+// PrefabAsset::Spawn (PrefabAsset.cpp:30-142) creates, with their world
+// matrices, the files their GUIDs name, the rebuild order and the light draws.
+class PrefabAsset {
+public:
+    PrefabAsset(const std::string& modsDir, const std::string& path) {
+        check(vx_prefab_load(modsDir.c_str(), path.c_str(), &h_));
+        uint32_t ne = 0, nf = 0;
+        check(vx_prefab_counts(h_, &ne, &nf));
+        entities_.resize(ne);
+        check(vx_prefab_entities(h_, entities_.data(), ne));
+        std::vector<char> buf(4096);
+        for (uint32_t i = 0; i < nf; ++i) {
+            check(vx_prefab_file(h_, i, buf.data(), buf.size()));
+            files_.emplace_back(buf.data());
+        }
+    }
+    ~PrefabAsset() { vx_prefab_destroy(h_); }
+    PrefabAsset(const PrefabAsset&) = delete;
+    PrefabAsset& operator=(const PrefabAsset&) = delete;
+
+    const std::vector<vx_prefab_entity>& Entities() const { return entities_; }
+    const std::vector<std::string>& Files() const { return files_; }
+    std::vector<uint32_t> RebuildOrder() const {
+        std::vector<uint32_t> idx(entities_.size());
+        uint32_t n = 0;
+        check(vx_prefab_rebuild_order(h_, idx.data(), (uint32_t)idx.size(), &n));
+        idx.resize(n);
+        return idx;
+    }
+    // Point lights, then spot lights, in the reference's draw order.
+    std::vector<vx_light> Lights() const {
+        uint32_t n = 0;
+        const vx_status s = vx_prefab_lights(h_, nullptr, 0, &n);  // count query: VX_ERR_RANGE when n > 0
+        if (s != VX_OK && s != VX_ERR_RANGE) check(s);
+        std::vector<vx_light> out(n);
+        if (n) check(vx_prefab_lights(h_, out.data(), n, &n));
+        return out;
+    }
+
+private:
+    vx_prefab* h_ = nullptr;
+    std::vector<vx_prefab_entity> entities_;
+    std::vector<std::string> files_;
+};
+
 using Entity = uint32_t;
 
 // Transform + VoxRenderer of one entity (Sources/World/Components.h:30-35,60-67).
@@ -95,8 +153,7 @@ struct VoxEntity {
 class ShadowVoxSystem {
 public:
     // Reference volume: 524 x 188 x 524 texels (ShadowVoxSystem.cpp:56-58).
-    explicit ShadowVoxSystem(Context& ctx, int sx = 524, int sy = 188, int sz = 524, int levels = 2)
-        : ctx_(ctx) {
+    explicit ShadowVoxSystem(Context& ctx, int sx = 524, int sy = 188, int sz = 524, int levels = 2) {
         check(vx_volume_create(ctx.get(), sx, sy, sz, levels, &vol_));
         check(vx_volume_clear(vol_, nullptr));
     }
@@ -139,13 +196,31 @@ public:
         entities_.erase(it);
     }
 
-    // OnUpdate (ShadowVoxSystem.cpp:116-201): every Changed entity in order, then
-    // the coarse levels over the dirty regions.
+    // The VoxRenderer entities of a spawned prefab in creation order, so the
+    // next OnUpdate visits them as the reference does. models maps a file
+    // index of the prefab to its loaded asset; entities whose Vox did not
+    // resolve are skipped (ShadowVoxSystem.cpp:120).
+    std::vector<Entity> Spawn(const PrefabAsset& pf, const std::map<int32_t, const VoxAsset*>& models) {
+        std::vector<Entity> ids;
+        const auto& ents = pf.Entities();
+        for (uint32_t i = 0; i < ents.size(); ++i) {
+            const vx_prefab_entity& e = ents[i];
+            if (!(e.flags & VX_ENT_VOX) || !(e.flags & VX_ENT_TRANSFORM) || e.model < 0) continue;
+            auto it = models.find(e.model);
+            if (it == models.end()) throw Error(VX_ERR_INVALID_ARGUMENT, "Spawn: a model of the prefab is not loaded");
+            ids.push_back(CreateEntity(it->second, e.world, e.pivot));
+        }
+        return ids;
+    }
+
+    // OnUpdate (ShadowVoxSystem.cpp:116-201): every Changed entity, newest
+    // first (EnTT 3.6 views walk their pools from the last element), then the
+    // coarse levels over the dirty regions.
     void OnUpdate(float /*dt*/, void* stream = nullptr) {
         std::vector<vx_instance> inst = std::move(pending_);
         pending_.clear();
-        for (auto& kv : entities_) {
-            VoxEntity& e = kv.second;
+        for (auto it = entities_.rbegin(); it != entities_.rend(); ++it) {
+            VoxEntity& e = it->second;
             if (!e.Changed) continue;
             if (e.Vox) inst.push_back(instance(e, VX_INST_UPDATE));
             e.Changed = false;
@@ -175,7 +250,6 @@ private:
         return in;
     }
 
-    Context& ctx_;
     vx_volume* vol_ = nullptr;
     std::map<Entity, VoxEntity> entities_;
     std::vector<vx_instance> pending_;

@@ -325,6 +325,89 @@ vx_status vx_shade_backward_batch(vx_ctx* ctx, const vx_shade_item* items, uint3
                                   const float* palette_dev, const float light[6], float* dpalette_dev,
                                   float* dlight_dev, void* stream);
 
+/* ---------------------------------------------------------------- asset files (host)
+ * Readers for the reference's on-disk formats. They read files and host
+ * memory only; vx_model_load also uploads the model it reads. */
+
+/* AssetGUID of a mod-relative path such as "default/FarmHouse/0.v": the hash
+ * of the path string (Assets::Hash, Assets.h:207-210) over the identifiers
+ * ModFolder maps (paths relative to Assets/Mods, Mod.h:86-94). The
+ * reference's files were written by a standard library whose string hash is
+ * 64-bit FNV-1a; this is that hash. */
+uint64_t vx_asset_guid(const char* path, size_t len);
+
+/* .v model (VoxAsset::Serialize, VoxAsset.h:42-50): u32 SizeX, SizeY, SizeZ
+ * (little-endian), then SizeX*SizeY*SizeZ material bytes, X fastest. Checks
+ * the header against nbytes (sizes 1..4096) and returns the dims and the
+ * offset of the voxel bytes (12). */
+vx_status vx_vox_parse(const uint8_t* bytes, size_t nbytes, int32_t dims[3], size_t* data_offset);
+/* Reads a .v file into a new model, uploads mip 0 and builds the mips
+ * (VoxAsset::OnLoad, VoxAsset.h:36-39). Sizes must be multiples of 4. */
+vx_status vx_model_load(vx_ctx* ctx, const char* path, vx_model** out, void* stream);
+
+/* .p palette (PalleteAsset::Serialize, PalleteAsset.h:63-69): 256 VoxMaterial
+ * of 7 bytes {r, g, b, a, roughness, metallic, emit}. Writes the 256 x 6 fp32
+ * palette the passes read, {albedo rgb, roughness, metallic, emissive}, each
+ * value byte/255 as the RGBA8 UNORM palette textures of
+ * PalleteCache::UploadPallete (PalleteCache.cpp:5-23) read. Host memory. */
+vx_status vx_palette_parse(const uint8_t* bytes, size_t nbytes, float* palette);
+vx_status vx_palette_load(const char* path, float* palette);
+
+/* .pf prefab (PrefabAsset::Spawn, PrefabAsset.cpp:30-142) spawned into a flat
+ * entity list, nested Instance prefabs expanded where they occur. GUIDs
+ * resolve through every file under mods_dir (the Assets/Mods folder); path
+ * is mod-relative ("default/FarmHouse.pf"). */
+typedef struct vx_prefab vx_prefab;
+enum { VX_ENT_TRANSFORM = 1, VX_ENT_VOX = 2, VX_ENT_LIGHT = 4, VX_ENT_INSTANCE = 8 };
+
+/* One spawned entity (Components.h Transform / VoxRenderer / Light). Matrices
+ * are column-major: matrix is Transform::Matrix, world the WorldMatrix
+ * TransformSystem computes (TransformSystem.cpp:36-70,124-135) in the
+ * reference's fp32 operation order. 256 bytes. */
+typedef struct {
+    int32_t id;                /* "Id" in the file that made the entity */
+    int32_t parent;            /* index of the parent in the list, -1 at the root */
+    int32_t flags;             /* VX_ENT_* */
+    int32_t model;             /* VoxRenderer.Vox as a file index, -1 if absent or not found */
+    int32_t palette;           /* VoxRenderer.Pallete as a file index, -1 if absent or not found */
+    int32_t light_type;        /* Light::Type (0 point, 1 spot, 2 directional, 3 ambient), -1 */
+    uint64_t vox_guid;
+    uint64_t palette_guid;
+    float position[3];
+    float rotation[3];         /* radians, applied Z, then Y, then X */
+    float scale[3];
+    float pivot[3];
+    float matrix[16];
+    float world[16];
+    float light_color[3];
+    float light_intensity;
+    float light_attenuation;
+    float light_range;
+    float light_angle;
+    float light_angle_attenuation;
+    int32_t pad[2];
+} vx_prefab_entity;
+
+vx_status vx_prefab_load(const char* mods_dir, const char* path, vx_prefab** out);
+vx_status vx_prefab_destroy(vx_prefab* p);
+/* Number of entities and of distinct referenced asset files (.v and .p). */
+vx_status vx_prefab_counts(const vx_prefab* p, uint32_t* nentities, uint32_t* nfiles);
+/* Entities in spawn order (the registry's creation order). cap >= nentities. */
+vx_status vx_prefab_entities(const vx_prefab* p, vx_prefab_entity* out, uint32_t cap);
+/* Full path (mods_dir + "/" + mod-relative path) of referenced file i. */
+vx_status vx_prefab_file(const vx_prefab* p, uint32_t i, char* buf, size_t cap);
+/* Entity indices of the VoxRenderer entities in the order
+ * ShadowVoxSystem::OnUpdate visits them on the frame after the spawn
+ * (ShadowVoxSystem.cpp:116-120): EnTT 3.6 views walk the smallest pool,
+ * here VoxRenderer's, from its newest element to its oldest. *n is the
+ * count even when it exceeds cap (then VX_ERR_RANGE). */
+vx_status vx_prefab_rebuild_order(const vx_prefab* p, uint32_t* idx, uint32_t cap, uint32_t* n);
+/* Point and spot lights as vx_light in the order the light passes draw them
+ * (WorldRenderer.cpp:243-257): the point pipeline's draws, then the spot
+ * pipeline's, each walking the Light pool newest first. position is
+ * WorldMatrix[3], direction WorldMatrix[2], color Color * Intensity. */
+vx_status vx_prefab_lights(const vx_prefab* p, vx_light* out, uint32_t cap, uint32_t* n);
+
 #ifdef __cplusplus
 }
 #endif

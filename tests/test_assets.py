@@ -1,0 +1,221 @@
+"""Asset loaders (.v, .p, .pf) on the reference's own asset data.
+
+The mod folder is rebuilt from tests/golden/prefabs.npz (the reference's
+ModernHouse / FarmHouse / player_ik prefabs, their .v models and .p
+palettes). Expected values come from the reference's files themselves (GUIDs
+written inside the prefabs must resolve to files), from an independent Python
+spawn in tests/golden/make_fixtures.py (world matrices, rebuild order,
+lights), and from the mathematics (palette values, rotation matrices).
+Host code only: no device is touched."""
+import json
+import os
+import struct
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from golden_mods import GOLDEN, write_mod_folder
+
+pytest.importorskip("torch")
+from voxelengine_amd import api, _lib as L  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def mods(tmp_path_factory):
+    return write_mod_folder(str(tmp_path_factory.mktemp("assets")))
+
+
+@pytest.fixture(scope="module")
+def z():
+    return np.load(GOLDEN)
+
+
+def test_guid_of_path_matches_reference_instance_guid(z):
+    # ModernHouse.pf spawns player_ik.pf by the GUID the reference wrote into it
+    pf = json.loads(z["pf_ModernHouse"].tobytes())
+    inst = [e["Instance"] for e in pf if "Instance" in e]
+    assert inst == ["D04CADEE517208D9"]
+    assert api.asset_guid("default/player_ik.pf") == 0xD04CADEE517208D9
+    assert api.asset_guid("") == 14695981039346656037  # FNV-1a offset basis
+
+
+@pytest.mark.parametrize("name", ["ModernHouse", "FarmHouse", "player_ik"])
+def test_every_guid_in_the_prefab_resolves(mods, z, name):
+    P = api.Prefab(mods, f"default/{name}.pf")
+    pf = json.loads(z[f"pf_{name}"].tobytes())
+    named = sum(1 for e in pf if "VoxRenderer" in e and int(e["VoxRenderer"]["Vox"], 16) != 0)
+    got = [e for e in P.entities if e.flags & L.ENT_VOX and e.vox_guid != 0]
+    assert len(got) >= named
+    assert all(e.model >= 0 for e in got), "a Vox GUID written by the reference did not resolve"
+    assert all(e.palette >= 0 for e in P.entities if e.flags & L.ENT_VOX and e.palette_guid != 0)
+    for f in P.files:
+        assert os.path.exists(f) and (f.endswith(".v") or f.endswith(".p"))
+
+
+@pytest.mark.parametrize("name", ["ModernHouse", "FarmHouse"])
+def test_world_matrices_and_models_match_fixture(mods, z, name):
+    """The prefab's own VoxRenderer entities (no nested ones) in file order
+    carry the fixture's instances: same .v file, world matrix, pivot."""
+    P = api.Prefab(mods, f"default/{name}.pf")
+    own = [i for i, e in enumerate(P.entities) if e.flags & L.ENT_VOX and not _nested(P, i)]
+    mi, mat, piv = z[f"{name}_inst_model"], z[f"{name}_inst_matrix"], z[f"{name}_inst_pivot"]
+    files = z[f"{name}_model_files"]
+    assert len(own) == len(mi)
+    for k, i in enumerate(own):
+        e = P.entities[i]
+        assert os.path.basename(P.files[e.model]) == str(files[mi[k]])
+        np.testing.assert_array_equal(P.world(i), mat[k])          # value equality (+0 == -0)
+        np.testing.assert_array_equal(np.array(e.pivot, np.float32), piv[k])
+
+
+def _nested(P, i):
+    while i >= 0:
+        if P.entities[i].flags & L.ENT_INSTANCE:
+            return True
+        i = P.entities[i].parent
+    return False
+
+
+def _rot64(axis, a):
+    c, s = np.cos(a), np.sin(a)
+    R = np.eye(4)
+    i, j = [(1, 2), (2, 0), (0, 1)][axis]
+    R[i, i], R[i, j], R[j, i], R[j, j] = c, -s, s, c
+    return R
+
+
+def _local64(e):
+    T = np.eye(4)
+    T[:3, 3] = np.array(e.position, np.float64)
+    r = np.array(e.rotation, np.float64)
+    return T @ _rot64(2, r[2]) @ _rot64(1, r[1]) @ _rot64(0, r[0]) @ np.diag(list(np.array(e.scale, np.float64)) + [1])
+
+
+def test_nested_player_matrices(mods, z):
+    """ModernHouse spawns player_ik under entity 0 with the outer Transform
+    (rotations about all three axes, non-unit scales). World = parent @ local,
+    checked against a float64 evaluation of the textbook rotations."""
+    P = api.Prefab(mods, "default/ModernHouse.pf")
+    pf_outer = json.loads(z["pf_ModernHouse"].tobytes())
+    pf_inner = json.loads(z["pf_player_ik"].tobytes())
+    assert len(P.entities) == len(pf_outer) - 1 + len(pf_inner)
+    roots = [i for i, e in enumerate(P.entities) if e.flags & L.ENT_INSTANCE]
+    assert len(roots) == 1
+    r = P.entities[roots[0]]
+    outer = next(e for e in pf_outer if "Instance" in e)
+    assert r.id == outer["Id"]
+    assert tuple(np.float32(v) for v in outer["Transform"]["Position"].split()) == tuple(r.position)
+    w64 = {}
+    for i, e in enumerate(P.entities):
+        par = w64.get(e.parent, np.eye(4)) if e.parent >= 0 else np.eye(4)
+        w64[i] = par @ _local64(e)
+        got = P.world(i).reshape(4, 4).T
+        np.testing.assert_allclose(got, w64[i], rtol=0, atol=2e-5)
+        m = np.array(e.matrix, np.float32).reshape(4, 4).T[:3, :3]
+        sc = np.array(e.scale, np.float64)
+        np.testing.assert_allclose((m / sc).T @ (m / sc), np.eye(3), atol=3e-6)   # rotation part orthonormal
+    nested = [i for i in range(len(P.entities)) if _nested(P, i)]
+    assert len(nested) == len(pf_inner)
+
+
+def test_farmhouse_rebuild_order_and_lights(mods, z):
+    P = api.Prefab(mods, "default/FarmHouse.pf")
+    order = P.rebuild_order()
+    ids = [P.entities[i].id for i in order if P.entities[i].model >= 0]
+    np.testing.assert_array_equal(ids, z["FarmHouse_ref_order_ids"])
+    assert order == sorted(order, reverse=True)     # newest VoxRenderer first
+    lights = P.lights()
+    ref = z["FarmHouse_ref_lights"]
+    assert len(lights) == len(ref) == 1
+    for k, l in enumerate(lights):
+        got = [l["type"], *l["position"], l["range"], *l["color"], l["attenuation"], *l["direction"], l["angle"],
+               l["angle_attenuation"]]
+        np.testing.assert_array_equal(np.array(got, np.float32), ref[k])
+
+
+def test_light_order_point_then_spot_newest_first(tmp_path):
+    """Two point lights and a spot light: the point draws come first, each
+    type newest first (EnTT 3.6 views walk pools backwards)."""
+    mods = tmp_path / "Mods"
+    (mods / "m").mkdir(parents=True)
+    ents = [{"Id": 0, "Name": "root", "Transform": {"Position": "0 0 0", "Rotation": "0 0 0", "Scale": "1 1 1"}}]
+    for k, t in enumerate([0, 1, 0]):
+        ents.append({"Id": k + 1, "Parent": 0, "Light": {"LightType": t, "Intensity": 2.0 + k, "Color": "1 0.5 0.25",
+                                                         "Attenuation": 2, "Range": 10 + k, "Angle": 0.3,
+                                                         "AngleAttenuation": 1},
+                     "Transform": {"Position": f"{k} 1 2", "Rotation": "0 0 0", "Scale": "1 1 1"}})
+    (mods / "m" / "lights.pf").write_text(json.dumps(ents))
+    P = api.Prefab(str(mods), "m/lights.pf")
+    ls = P.lights()
+    assert [int(l["type"]) for l in ls] == [0, 0, 1]
+    assert [float(l["range"]) for l in ls] == [12.0, 10.0, 11.0]
+    np.testing.assert_array_equal(ls[0]["color"], np.float32([1, 0.5, 0.25]) * np.float32(4.0))
+    np.testing.assert_array_equal(ls[2]["position"], [1, 1, 2])
+    np.testing.assert_array_equal(ls[2]["direction"], [0, 0, 1])   # WorldMatrix[2] of an unrotated entity
+
+
+@pytest.mark.parametrize("folder", ["ModernHouse", "FarmHouse", "Player"])
+def test_palette_values_are_nearest_floats(mods, z, folder):
+    raw = z[f"palette_{folder}"]
+    assert raw.size == 256 * 7
+    pal = api.load_palette(os.path.join(mods, "default", folder, f"{folder}.p"))
+    np.testing.assert_array_equal(pal, api.parse_palette(raw.tobytes()))
+    src = raw.reshape(256, 7)[:, [0, 1, 2, 4, 5, 6]]          # r g b (a skipped) roughness metallic emit
+    for k in np.unique(src):
+        v = pal[src == k]
+        assert np.all(v == v[0])
+        x = Fraction(int(k), 255)
+        f = np.float32(v[0])
+        lo, hi = np.nextafter(f, np.float32(-1)), np.nextafter(f, np.float32(2))
+        assert abs(Fraction(float(f)) - x) <= abs(Fraction(float(lo)) - x)
+        assert abs(Fraction(float(f)) - x) <= abs(Fraction(float(hi)) - x)
+
+
+def test_vox_parse_and_errors(z):
+    m = z["FarmHouse_model1"]
+    sz, sy, sx = m.shape
+    data = struct.pack("<3I", sx, sy, sz) + m.tobytes()
+    dims, vox = api.parse_vox(data)
+    assert dims == (sx, sy, sz)
+    np.testing.assert_array_equal(vox, m)
+    with pytest.raises(L.VXError):
+        api.parse_vox(data[:-1])
+    with pytest.raises(L.VXError):
+        api.parse_vox(data[:11])
+    with pytest.raises(L.VXError):
+        api.parse_vox(struct.pack("<3I", 0, 4, 4))
+    with pytest.raises(L.VXError):
+        api.parse_vox(struct.pack("<3I", 8192, 4, 4) + bytes(8192 * 16))
+    with pytest.raises(L.VXError):
+        api.parse_palette(bytes(1791))
+
+
+def test_prefab_errors(tmp_path):
+    mods = tmp_path / "Mods"
+    (mods / "m").mkdir(parents=True)
+    with pytest.raises(L.VXError, match="cannot read"):
+        api.Prefab(str(mods), "m/missing.pf")
+    (mods / "m" / "bad.pf").write_text('[{"Id": 0,]')
+    with pytest.raises(L.VXError, match="JSON"):
+        api.Prefab(str(mods), "m/bad.pf")
+    (mods / "m" / "vec.pf").write_text('[{"Id": 0, "Transform": {"Position": "1 2", "Rotation": "0 0 0", '
+                                      '"Scale": "1 1 1"}}]')
+    with pytest.raises(L.VXError, match="Transform"):
+        api.Prefab(str(mods), "m/vec.pf")
+    (mods / "m" / "inst.pf").write_text('[{"Id": 0, "Instance": "0123456789ABCDEF"}]')
+    with pytest.raises(L.VXError, match="Instance"):
+        api.Prefab(str(mods), "m/inst.pf")
+    (mods / "m" / "cycle.pf").write_text('[{"Id": 0, "Instance": "%016X"}]' % api.asset_guid("m/cycle.pf"))
+    with pytest.raises(L.VXError, match="nest"):
+        api.Prefab(str(mods), "m/cycle.pf")
+    (mods / "m" / "parent.pf").write_text('[{"Id": 0}, {"Id": 1, "Parent": 5}]')
+    with pytest.raises(L.VXError, match="Parent"):
+        api.Prefab(str(mods), "m/parent.pf")
+    # a VoxRenderer whose files are absent keeps model -1 and is skipped by instances()
+    (mods / "m" / "novox.pf").write_text('[{"Id": 0, "Transform": {"Position": "0 0 0", "Rotation": "0 0 0", '
+                                        '"Scale": "1 1 1"}, "VoxRenderer": {"Pallete": "0", "Vox": "ABC"}}]')
+    P = api.Prefab(str(mods), "m/novox.pf")
+    assert P.entities[0].model == -1 and P.entities[0].vox_guid == 0xABC
+    assert P.rebuild_order() == [0]
+    assert P.instances({}) == []

@@ -94,6 +94,18 @@ class TraceTuning(C.Structure):
                 ("steps", C.c_int32), ("pad", C.c_int32 * 3)]
 
 
+class PrefabEntity(C.Structure):
+    _fields_ = [("id", C.c_int32), ("parent", C.c_int32), ("flags", C.c_int32), ("model", C.c_int32),
+                ("palette", C.c_int32), ("light_type", C.c_int32), ("vox_guid", C.c_uint64),
+                ("palette_guid", C.c_uint64), ("position", C.c_float * 3), ("rotation", C.c_float * 3),
+                ("scale", C.c_float * 3), ("pivot", C.c_float * 3), ("matrix", C.c_float * 16),
+                ("world", C.c_float * 16), ("light_color", C.c_float * 3), ("light_intensity", C.c_float),
+                ("light_attenuation", C.c_float), ("light_range", C.c_float), ("light_angle", C.c_float),
+                ("light_angle_attenuation", C.c_float), ("pad", C.c_int32 * 2)]
+
+
+ENT_TRANSFORM, ENT_VOX, ENT_LIGHT, ENT_INSTANCE = 1, 2, 4, 8
+
 # Every symbol the header declares, with its signature.
 SIGNATURES = {
     "vx_api_version": ([], C.c_int32),
@@ -138,6 +150,18 @@ SIGNATURES = {
                       C.c_int32),
     "vx_shade_backward_batch": ([P, C.POINTER(ShadeItem), C.c_uint32, P, C.POINTER(C.c_float), P, P, P],
                                 C.c_int32),
+    "vx_asset_guid": ([C.c_char_p, C.c_size_t], C.c_uint64),
+    "vx_vox_parse": ([P, C.c_size_t, C.POINTER(C.c_int32), C.POINTER(C.c_size_t)], C.c_int32),
+    "vx_model_load": ([P, C.c_char_p, C.POINTER(P), P], C.c_int32),
+    "vx_palette_parse": ([P, C.c_size_t, C.POINTER(C.c_float)], C.c_int32),
+    "vx_palette_load": ([C.c_char_p, C.POINTER(C.c_float)], C.c_int32),
+    "vx_prefab_load": ([C.c_char_p, C.c_char_p, C.POINTER(P)], C.c_int32),
+    "vx_prefab_destroy": ([P], C.c_int32),
+    "vx_prefab_counts": ([P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int32),
+    "vx_prefab_entities": ([P, C.POINTER(PrefabEntity), C.c_uint32], C.c_int32),
+    "vx_prefab_file": ([P, C.c_uint32, C.c_char_p, C.c_size_t], C.c_int32),
+    "vx_prefab_rebuild_order": ([P, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int32),
+    "vx_prefab_lights": ([P, C.POINTER(Light), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int32),
 }
 
 _lib = None

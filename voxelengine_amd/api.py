@@ -351,3 +351,118 @@ def light_pass(ctx, vol, vd, gbuf, palette, lights_dev, nlights, rgb, point_marc
     check(L.lib().vx_light_pass(ctx.h, vol.h, C.byref(v), C.byref(g), _ptr(palette),
                                 _ptr(lights_dev) if nlights else None, nlights, C.byref(p), _ptr(rgb),
                                 _ptr(fetches), _stream(stream)))
+
+
+# ---------------------------------------------------------------- asset files (host)
+def asset_guid(path: str) -> int:
+    """AssetGUID of a mod-relative path ("default/FarmHouse/0.v")."""
+    b = path.encode()
+    return int(L.lib().vx_asset_guid(b, len(b)))
+
+
+def parse_vox(data: bytes):
+    """.v bytes -> ((sx, sy, sz), uint8 array (sz, sy, sx))."""
+    buf = np.frombuffer(data, np.uint8)
+    d = (C.c_int32 * 3)()
+    off = C.c_size_t()
+    check(L.lib().vx_vox_parse(_hp(buf) if buf.size else None, buf.size, d, C.byref(off)))
+    dims = (d[0], d[1], d[2])
+    vox = buf[off.value:off.value + dims[0] * dims[1] * dims[2]].reshape(dims[2], dims[1], dims[0])
+    return dims, vox
+
+
+def parse_palette(data: bytes) -> np.ndarray:
+    """.p bytes -> 256 x 6 fp32 palette {albedo rgb, roughness, metallic, emissive}."""
+    buf = np.frombuffer(data, np.uint8)
+    out = np.empty((256, 6), np.float32)
+    check(L.lib().vx_palette_parse(_hp(buf) if buf.size else None, buf.size,
+                                   out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
+def load_palette(path: str) -> np.ndarray:
+    out = np.empty((256, 6), np.float32)
+    check(L.lib().vx_palette_load(str(path).encode(), out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
+def load_model(ctx: Context, path: str, stream=None) -> Model:
+    """A .v file as a Model with its mips built."""
+    h = C.c_void_p()
+    check(L.lib().vx_model_load(ctx.h, str(path).encode(), C.byref(h), _stream(stream)))
+    m = Model.__new__(Model)
+    m.h, m.ctx = h, ctx
+    d = (C.c_int32 * 3)()
+    check(L.lib().vx_model_dims(h, d))
+    m.dims = (d[0], d[1], d[2])
+    return m
+
+
+def _mat(a) -> np.ndarray:
+    return np.ctypeslib.as_array(a).astype(np.float32).copy()
+
+
+class Prefab:
+    """A spawned .pf prefab: entities in creation order, the files they
+    reference, the rebuild order and the light list of the reference's ECS."""
+
+    def __init__(self, mods_dir: str, path: str):
+        h = C.c_void_p()
+        check(L.lib().vx_prefab_load(str(mods_dir).encode(), str(path).encode(), C.byref(h)))
+        self.h = h
+        ne, nf = C.c_uint32(), C.c_uint32()
+        check(L.lib().vx_prefab_counts(h, C.byref(ne), C.byref(nf)))
+        arr = (L.PrefabEntity * max(ne.value, 1))()
+        check(L.lib().vx_prefab_entities(h, arr, ne.value))
+        self.entities = list(arr)[:ne.value]
+        buf = C.create_string_buffer(4096)
+        self.files = []
+        for i in range(nf.value):
+            check(L.lib().vx_prefab_file(h, i, buf, len(buf)))
+            self.files.append(buf.value.decode())
+
+    def close(self):
+        if self.h:
+            L.lib().vx_prefab_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def world(self, i: int) -> np.ndarray:
+        return _mat(self.entities[i].world)
+
+    def rebuild_order(self) -> list:
+        n = C.c_uint32()
+        idx = (C.c_uint32 * max(len(self.entities), 1))()
+        check(L.lib().vx_prefab_rebuild_order(self.h, idx, len(idx), C.byref(n)))
+        return [int(idx[k]) for k in range(n.value)]
+
+    def lights(self) -> np.ndarray:
+        n = C.c_uint32()
+        out = np.zeros(max(len(self.entities), 1), LIGHT_DTYPE)
+        check(L.lib().vx_prefab_lights(self.h, out.ctypes.data_as(C.POINTER(L.Light)), out.size, C.byref(n)))
+        return out[:n.value].copy()
+
+    def load_models(self, ctx: Context, stream=None) -> dict:
+        """Every referenced .v file as a Model, keyed by file index."""
+        idx = sorted({e.model for e in self.entities if e.flags & L.ENT_VOX and e.model >= 0})
+        return {i: load_model(ctx, self.files[i], stream) for i in idx}
+
+    def instances(self, models: dict, prev=None, order=None) -> list:
+        """vx_instance list for Volume.rebuild in `order` (default: the
+        reference's rebuild order); prev: per-entity previous world matrices
+        (default identity, as for entities spawned this frame). Entities whose
+        Vox did not resolve are skipped, as ShadowVoxSystem does."""
+        out = []
+        for i in (self.rebuild_order() if order is None else order):
+            e = self.entities[i]
+            if e.model < 0:
+                continue
+            m = models[e.model]
+            p = IDENTITY if prev is None else prev[i]
+            out.append(make_instance(m.mip_ptr(0), m.dims, p, self.world(i), tuple(e.pivot)))
+        return out

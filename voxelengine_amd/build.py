@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvx_amd.so")
-SOURCES = ["vx_kernels.hip", "vx_api.hip"]
+SOURCES = ["vx_kernels.hip", "vx_api.hip", "vx_assets.cpp"]
 ARCH = "gfx950"
 
 HIP_FLAGS = [

@@ -34,6 +34,12 @@ static vx_status fail(vx_status code, const char* fmt, ...) {
     return code;
 }
 
+// The host-only sources (asset loaders) report through the same per-thread message.
+vx_status vx_set_error(vx_status code, const char* msg) {
+    g_last_error = msg;
+    return code;
+}
+
 #define VX_HIP(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
