@@ -183,6 +183,51 @@ def bench_large(ctx, pal_d, reps=3):
     return out
 
 
+def bench_prefabs(ctx, reps=5):
+    """The reference's FarmHouse and ModernHouse prefabs (files rebuilt from
+    tests/golden/prefabs.npz) through the loaders into the 524x188x524 volume:
+    spawn (host), model loads (read + upload + mips) and the first-frame
+    rebuild in the reference's order, timed on the host around each call."""
+    import tempfile
+    import torch
+    from voxelengine_amd import api
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_mods import write_mod_folder
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        mods = write_mod_folder(tmp)
+        for name in ("FarmHouse", "ModernHouse"):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                P = api.Prefab(mods, f"default/{name}.pf")
+            spawn_ms = (time.perf_counter() - t0) / reps * 1e3
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            models = P.load_models(ctx)
+            torch.cuda.synchronize()
+            load_ms = (time.perf_counter() - t0) * 1e3
+            insts = P.instances(models)
+            vol = api.Volume(ctx, 524, 188, 524, nlevels=2)
+            times = []
+            for _ in range(reps):
+                vol.clear()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                vol.rebuild(insts)
+                torch.cuda.synchronize()
+                times.append((time.perf_counter() - t0) * 1e3)
+            voxels = sum(int(np.prod(m.dims)) for m in (models[P.entities[i].model] for i in P.rebuild_order()
+                                                        if P.entities[i].model >= 0))
+            res[name] = {"entities": len(P.entities), "instances": len(insts), "model_voxels": voxels,
+                         "spawn_ms": round(spawn_ms, 3), "model_load_ms": round(load_ms, 2),
+                         "rebuild_ms": round(min(times), 3)}
+            vol.close()
+            for m in models.values():
+                m.close()
+            P.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,6 +382,8 @@ def main():
 
     if not args.no_extras:
         out["batched"] = bench_batched(ctx, pal_d)
+        log("prefabs ...")
+        out["prefabs"] = bench_prefabs(ctx)
         log("config L ...")
         out["large_batched"] = bench_large(ctx, pal_d)
 
