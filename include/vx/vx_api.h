@@ -226,6 +226,35 @@ vx_status vx_volume_download_level(const vx_volume* v, int32_t level, uint64_t* 
  * world model with an identity WorldMatrix. */
 vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_view* view, const vx_gbuf* gbuf,
                      void* stream);
+/* One voxel draw of the geometry pass (GeometryVoxelPipeline::Cmd,
+ * GeometryVoxelPipeline.h:29-35, filled by WorldRenderer.cpp:210-220):
+ * world is the entity's WorldMatrix translated by -Pivot, last_world the
+ * PreviousWorldMatrix translated the same way. 144 bytes. */
+typedef struct {
+    const vx_model* model;
+    float world[16];
+    float last_world[16];
+    int32_t volume_rid;    /* VolumeRID: scales the fragment depth by 1 - 1e-7 * rid */
+    int32_t palette;       /* PalleteIndex, written to vx_gbuf_ext.palette */
+} vx_vox_cmd;
+
+/* Optional geometry-pass outputs beyond vx_gbuf (device, NULL = not written). */
+typedef struct {
+    float* motion;         /* width*height*2: out_Motion (GeometryVoxel.frag:171) */
+    float* depth;          /* width*height: the D24 depth as read back, q / (2^24 - 1); 1 = no hit */
+    uint16_t* palette;     /* width*height: the hit's PalleteIndex (0 = no hit) */
+} vx_gbuf_ext;
+
+/* The geometry pass over n voxel draws (GeometryVoxelPipeline::Use,
+ * GeometryVoxelPipeline.h:48-70; GeometryVoxel.vert:62-78,
+ * GeometryVoxel.frag:127-183): each pixel's camera ray is marched through
+ * every draw whose box covers it, in model space with the model's mips, and
+ * the D24 depth test (LESS, draws in array order) keeps the nearest hit.
+ * gbuf.pos is the world hit point in fine-voxel units (world * 10), gbuf.normal
+ * the world normal. n <= 4096 (MAX_INSTANCES, GeometryVoxelPipeline.h:17). */
+vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, const vx_view* view,
+                               const vx_gbuf* gbuf, const vx_gbuf_ext* ext, void* stream);
+
 /* Shadow + AO + spec-occ rays: LightAmbientPipeline::Use (LightAmbientPipeline.h:35)
  * and LightReflectionPipeline::Use (LightReflectionPipeline.h:34) ray set-up and
  * marching. palette_dev: 256*6 floats {albedo rgb, roughness, metallic, emissive}. */
@@ -407,6 +436,15 @@ vx_status vx_prefab_rebuild_order(const vx_prefab* p, uint32_t* idx, uint32_t ca
  * pipeline's, each walking the Light pool newest first. position is
  * WorldMatrix[3], direction WorldMatrix[2], color Color * Intensity. */
 vx_status vx_prefab_lights(const vx_prefab* p, vx_light* out, uint32_t cap, uint32_t* n);
+/* The geometry pass's draws for the spawned prefab (WorldRenderer.cpp:210-220):
+ * entities with a resolved Vox and Pallete, in the owning group's order
+ * (newest VoxRenderer first). world = WorldMatrix translated by -Pivot;
+ * last_world = PreviousWorldMatrix (prev[16*entity], or the identity a spawned
+ * entity starts with when prev is NULL) translated likewise. models[i] is the
+ * loaded model of file i (NULL for other files); volume_rid = file index + 1,
+ * palette = the palette's file index. *n is the count even above cap. */
+vx_status vx_prefab_vox_cmds(const vx_prefab* p, const vx_model* const* models, uint32_t nmodels,
+                             const float* prev, vx_vox_cmd* out, uint32_t cap, uint32_t* n);
 
 #ifdef __cplusplus
 }

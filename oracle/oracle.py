@@ -283,6 +283,35 @@ def primary(mips, vd: dict, nthreads=0):
     return pos, nrm, mat
 
 
+class VoxCmd(C.Structure):
+    _fields_ = [("mips", C.c_void_p * 3), ("dims", C.c_int32 * 3), ("world", C.c_float * 16),
+                ("last_world", C.c_float * 16), ("rid", C.c_int32), ("palette", C.c_int32)]
+
+
+def primary_instances(cmds, vd: dict, nthreads=0):
+    """cmds: list of dict(mips=[3 uint8 arrays (sz,sy,sx)], world, last_world, rid, palette)."""
+    arr = (VoxCmd * max(len(cmds), 1))()
+    keep = []
+    for k, c in enumerate(cmds):
+        mips = [np.ascontiguousarray(m, np.uint8) for m in c["mips"]]
+        keep.append(mips)
+        arr[k].mips = (C.c_void_p * 3)(*[m.ctypes.data for m in mips])
+        sz, sy, sx = mips[0].shape
+        arr[k].dims = (C.c_int32 * 3)(sx, sy, sz)
+        arr[k].world = (C.c_float * 16)(*np.asarray(c["world"], np.float32).reshape(16))
+        lw = c.get("last_world", c["world"])
+        arr[k].last_world = (C.c_float * 16)(*np.asarray(lw, np.float32).reshape(16))
+        arr[k].rid, arr[k].palette = int(c.get("rid", 1)), int(c.get("palette", 0))
+    W, H = vd["width"], vd["height"]
+    pos = np.zeros((H * W, 3), np.float32); nrm = np.zeros((H * W, 3), np.float32)
+    mat = np.zeros(H * W, np.uint8); motion = np.zeros((H * W, 2), np.float32)
+    depth = np.zeros(H * W, np.float32); pal = np.zeros(H * W, np.uint16)
+    v = view_struct(vd)
+    lib().vxo_primary_instances(arr, len(cmds), C.byref(v), _p(pos), _p(nrm), _p(mat), _p(motion), _p(depth),
+                                _p(pal), nthreads)
+    return dict(pos=pos, nrm=nrm, mat=mat, motion=motion, depth=depth, palette=pal)
+
+
 def trace_params(p: dict) -> TraceParams:
     t = TraceParams()
     t.shadow_march = p["shadow_march"]; t.ao_march = p["ao_march"]; t.spec_march = p["spec_march"]

@@ -531,6 +531,143 @@ void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_
     (void)nthreads;
 }
 
+/* ---- multi-instance geometry pass (next rank 3) ---- */
+/* 4x4 inverse (column-major) in float64: cofactors over the determinant. */
+static int inverse4(const double* m, double* inv) {
+    double c[16];
+    c[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+           m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    c[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    c[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+           m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    c[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    c[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    c[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+           m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    c[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    c[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    c[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+           m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    c[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+           m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    c[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+            m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    c[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    c[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+           m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    c[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+           m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    c[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+            m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    c[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+            m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    double det = m[0] * c[0] + m[1] * c[4] + m[2] * c[8] + m[3] * c[12];
+    if (det == 0.0 || !isfinite(det)) return 0;
+    for (int i = 0; i < 16; ++i) inv[i] = c[i] / det;
+    return 1;
+}
+
+static void mat_mul_f(const float* a, const float* b, float* out) {
+    for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 4; ++r)
+            out[j * 4 + r] = ((a[0 + r] * b[j * 4 + 0] + a[4 + r] * b[j * 4 + 1]) + a[8 + r] * b[j * 4 + 2]) +
+                             a[12 + r] * b[j * 4 + 3];
+}
+
+/* The geometry pass per pixel: GeometryVoxel.vert:72-75 gives the camera in
+ * model space (inverse(WorldMatrix) * camera * 10) and the direction toward
+ * the fragment; GeometryVoxel.frag:131-146 clips to the box and marches; the
+ * hit's fragment depth (frag:176-177) goes through the D24 conversion and the
+ * LESS test (evk.cpp:481) against the cleared 1.0, draws in order. */
+void vxo_primary_instances(const vxo_vox_cmd* cmds, int32_t n, const vxo_view* vw, float* pos, float* nrm,
+                           uint8_t* mat, float* motion, float* depth, uint16_t* palette, int32_t nthreads) {
+    const int32_t W = vw->width, H = vw->height;
+    float (*winv)[16] = (float (*)[16])malloc(sizeof(float) * 16 * (size_t)(n > 0 ? n : 1));
+    float (*lc)[3] = (float (*)[3])malloc(sizeof(float) * 3 * (size_t)(n > 0 ? n : 1));
+    int* ok = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int32_t k = 0; k < n; ++k) {
+        double wd[16], wi[16];
+        for (int q = 0; q < 16; ++q) wd[q] = cmds[k].world[q];
+        ok[k] = inverse4(wd, wi);
+        for (int q = 0; q < 16; ++q) winv[k][q] = (float)wi[q];
+        double cx = vw->cam_pos[0], cy = vw->cam_pos[1], cz = vw->cam_pos[2];
+        for (int q = 0; q < 3; ++q) lc[k][q] = (float)((((wi[q] * cx + wi[4 + q] * cy) + wi[8 + q] * cz) + wi[12 + q]) * 10.0);
+    }
+    float pv[16], plv[16];
+    mat_mul_f(vw->proj, vw->view, pv);
+    mat_mul_f(vw->proj, vw->last_view, plv);
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+    for (int32_t py = 0; py < H; ++py) {
+        for (int32_t px = 0; px < W; ++px) {
+            size_t i = (size_t)py * W + px;
+            float dw[3], ndc[2];
+            pixel_ray(vw, px, py, dw, ndc);
+            uint32_t best_q = 16777215u;
+            int32_t best = -1;
+            float bhit[3] = {0, 0, 0}, bn[3] = {0, 0, 0};
+            uint8_t bmat = 0;
+            for (int32_t k = 0; k < n; ++k) {
+                if (!ok[k]) continue;
+                const float* Wi = winv[k];
+                float dl[3], dr[3], start[3], hit[3], nn[3];
+                dr[0] = (Wi[0] * dw[0] + Wi[4] * dw[1]) + Wi[8] * dw[2];
+                dr[1] = (Wi[1] * dw[0] + Wi[5] * dw[1]) + Wi[9] * dw[2];
+                dr[2] = (Wi[2] * dw[0] + Wi[6] * dw[1]) + Wi[10] * dw[2];
+                normalize3(dr, dl);
+                float size[3] = {(float)cmds[k].dims[0], (float)cmds[k].dims[1], (float)cmds[k].dims[2]};
+                vxo_clip_to_aabb(lc[k], dl, size, start);
+                uint8_t m = 0;
+                if (!vxo_intersect_volume(cmds[k].mips, cmds[k].dims, start, dl, lc[k], ndc, vw->res, vw->frame,
+                                          hit, nn, &m))
+                    continue;
+                float h4[4] = {hit[0] * 0.1f, hit[1] * 0.1f, hit[2] * 0.1f, 1.0f}, wh[4], cur[4];
+                mat_vec(cmds[k].world, h4, wh);
+                mat_vec(pv, wh, cur);
+                float d = ((1.0f - 1e-7f * (float)cmds[k].rid) * (cur[3] - 0.1f)) / (4096.0f - 0.1f);
+                d = d < 0.0f ? 0.0f : (d > 1.0f ? 1.0f : d);
+                uint32_t q = (uint32_t)rintf(d * 16777215.0f);
+                if (q < best_q) {
+                    best_q = q; best = k; bmat = m;
+                    for (int c = 0; c < 3; ++c) { bhit[c] = hit[c]; bn[c] = nn[c]; }
+                }
+            }
+            float po[3] = {0, 0, 0}, no[3] = {0, 0, 0}, mo[2] = {0, 0};
+            uint16_t pal = 0;
+            if (best >= 0) {
+                const vxo_vox_cmd* c = &cmds[best];
+                float h4[4] = {bhit[0] * 0.1f, bhit[1] * 0.1f, bhit[2] * 0.1f, 1.0f}, wh[4], cur[4], lw[4], last[4];
+                mat_vec(c->world, h4, wh);
+                mat_vec(pv, wh, cur);
+                mat_vec(c->last_world, h4, lw);
+                mat_vec(plv, lw, last);
+                float n4[4] = {bn[0], bn[1], bn[2], 0.0f}, wn[4];
+                mat_vec(c->world, n4, wn);
+                float l = sqrtf(((wn[0] * wn[0] + wn[1] * wn[1]) + wn[2] * wn[2]) + wn[3] * wn[3]);
+                for (int k = 0; k < 3; ++k) { po[k] = wh[k] * 10.0f; no[k] = wn[k] / l; }
+                mo[0] = -0.5f * (cur[0] / cur[3] - last[0] / last[3]);
+                mo[1] = 0.5f * (cur[1] / cur[3] - last[1] / last[3]);
+                pal = (uint16_t)c->palette;
+            }
+            for (int k = 0; k < 3; ++k) { pos[3 * i + k] = po[k]; nrm[3 * i + k] = no[k]; }
+            mat[i] = bmat;
+            if (motion) { motion[2 * i] = mo[0]; motion[2 * i + 1] = mo[1]; }
+            if (depth) depth[i] = (float)best_q / 16777215.0f;
+            if (palette) palette[i] = pal;
+        }
+    }
+    free(winv); free(lc); free(ok);
+    (void)nthreads;
+}
+
 /* Tangent frame of calculateAmbientIrradiance, LightAmbient.frag:112-113. */
 static inline void tangent_frame(const float n[3], float t[3], float b[3]) {
     if (fabsf(n[2]) > 0.5f) { t[0] = 0.0f; t[1] = -n[2]; t[2] = n[1]; }

@@ -127,6 +127,21 @@ int32_t vxo_intersect_volume(const uint8_t* const mips[3], const int32_t dims[3]
 /* ---- full passes ---- */
 void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* view,
                  float* pos, float* nrm, uint8_t* mat, int32_t nthreads);
+/* One voxel draw (GeometryVoxelPipeline::Cmd, GeometryVoxelPipeline.h:29-35)
+ * with its model's mips. */
+typedef struct {
+    const uint8_t* mips[3];
+    int32_t dims[3];
+    float world[16];
+    float last_world[16];
+    int32_t rid;
+    int32_t palette;
+} vxo_vox_cmd;
+/* The geometry pass over n draws (GeometryVoxel.vert:62-78, GeometryVoxel.frag:
+ * 127-183, D24 LESS depth test in draw order); every draw is tried at every
+ * pixel. motion / depth / palette may be NULL. */
+void vxo_primary_instances(const vxo_vox_cmd* cmds, int32_t n, const vxo_view* view, float* pos, float* nrm,
+                           uint8_t* mat, float* motion, float* depth, uint16_t* palette, int32_t nthreads);
 /* pixels: optional list of pixel indices to process (NULL = all). */
 void vxo_trace(const vxo_occ* v, const vxo_view* view, const vxo_trace_params* p,
                const float* pos, const float* nrm, const uint8_t* mat, const float* palette,

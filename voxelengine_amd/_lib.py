@@ -94,6 +94,15 @@ class TraceTuning(C.Structure):
                 ("steps", C.c_int32), ("pad", C.c_int32 * 3)]
 
 
+class VoxCmd(C.Structure):
+    _fields_ = [("model", P), ("world", C.c_float * 16), ("last_world", C.c_float * 16), ("volume_rid", C.c_int32),
+                ("palette", C.c_int32)]
+
+
+class GBufExt(C.Structure):
+    _fields_ = [("motion", P), ("depth", P), ("palette", P)]
+
+
 class PrefabEntity(C.Structure):
     _fields_ = [("id", C.c_int32), ("parent", C.c_int32), ("flags", C.c_int32), ("model", C.c_int32),
                 ("palette", C.c_int32), ("light_type", C.c_int32), ("vox_guid", C.c_uint64),
@@ -162,6 +171,10 @@ SIGNATURES = {
     "vx_prefab_file": ([P, C.c_uint32, C.c_char_p, C.c_size_t], C.c_int32),
     "vx_prefab_rebuild_order": ([P, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int32),
     "vx_prefab_lights": ([P, C.POINTER(Light), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int32),
+    "vx_prefab_vox_cmds": ([P, C.POINTER(P), C.c_uint32, C.POINTER(C.c_float), C.POINTER(VoxCmd), C.c_uint32,
+                            C.POINTER(C.c_uint32)], C.c_int32),
+    "vx_primary_instances": ([P, C.POINTER(VoxCmd), C.c_uint32, C.POINTER(View), C.POINTER(GBuf),
+                              C.POINTER(GBufExt), P], C.c_int32),
 }
 
 _lib = None

@@ -232,6 +232,39 @@ class GBuffer:
         return L.GBuf(_ptr(self.pos), _ptr(self.normal), _ptr(self.material))
 
 
+class GBufferExt:
+    """Optional geometry-pass outputs: motion (2 per pixel), D24 depth read back
+    as float, palette index."""
+
+    def __init__(self, width, height, device="cuda"):
+        n = width * height
+        self.motion = torch.zeros((n, 2), dtype=torch.float32, device=device)
+        self.depth = torch.zeros(n, dtype=torch.float32, device=device)
+        self.palette = torch.zeros(n, dtype=torch.int16, device=device)
+
+    def struct(self) -> L.GBufExt:
+        return L.GBufExt(_ptr(self.motion), _ptr(self.depth), _ptr(self.palette))
+
+
+def make_vox_cmd(model: "Model", world, last_world=None, volume_rid=1, palette=0) -> L.VoxCmd:
+    c = L.VoxCmd()
+    c.model = model.h
+    c.world = L.f16(*np.asarray(world, np.float32).reshape(16))
+    lw = world if last_world is None else last_world
+    c.last_world = L.f16(*np.asarray(lw, np.float32).reshape(16))
+    c.volume_rid, c.palette = int(volume_rid), int(palette)
+    return c
+
+
+def primary_instances(ctx: Context, cmds, vd: dict, gbuf: GBuffer, ext: GBufferExt = None, stream=None):
+    """The geometry pass over voxel draws (list of L.VoxCmd) in draw order."""
+    arr = (L.VoxCmd * max(len(cmds), 1))(*cmds)
+    v = view_struct(vd); g = gbuf.struct()
+    e = ext.struct() if ext is not None else None
+    check(L.lib().vx_primary_instances(ctx.h, arr, len(cmds), C.byref(v), C.byref(g),
+                                       C.byref(e) if e is not None else None, _stream(stream)))
+
+
 class Visibility:
     def __init__(self, width, height, fetches=False, device="cuda"):
         n = width * height
@@ -451,6 +484,22 @@ class Prefab:
         """Every referenced .v file as a Model, keyed by file index."""
         idx = sorted({e.model for e in self.entities if e.flags & L.ENT_VOX and e.model >= 0})
         return {i: load_model(ctx, self.files[i], stream) for i in idx}
+
+    def vox_cmds(self, models: dict, prev=None) -> list:
+        """The geometry pass's draws (L.VoxCmd) in the reference's order; models:
+        file index -> Model; prev: per-entity previous world matrices or None."""
+        nf = len(self.files)
+        arr = (C.c_void_p * max(nf, 1))()
+        for fi, m in models.items():
+            arr[fi] = m.h.value if hasattr(m.h, "value") else m.h
+        pv = None
+        if prev is not None:
+            pv = np.ascontiguousarray(np.asarray(prev, np.float32).reshape(len(self.entities), 16))
+        n = C.c_uint32()
+        out = (L.VoxCmd * max(len(self.entities), 1))()
+        check(L.lib().vx_prefab_vox_cmds(self.h, arr, nf, pv.ctypes.data_as(C.POINTER(C.c_float)) if pv is not None
+                                         else None, out, len(out), C.byref(n)))
+        return list(out)[:n.value]
 
     def instances(self, models: dict, prev=None, order=None) -> list:
         """vx_instance list for Volume.rebuild in `order` (default: the

@@ -95,6 +95,12 @@ struct vx_ctx {
     vx_trace_tuning tuning;
     bool profile;            // record events around march kernels
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;
+    InstConst* inst;         // multi-instance primary: per-draw constants
+    int32_t* inst_rects;     // and screen rectangles (device)
+    uint32_t inst_cap;
+    uint16_t* tile_lists;    // per-tile instance lists
+    uint32_t* tile_counts;
+    uint32_t tile_cap;
 };
 
 static void profile_clear(vx_ctx* c) {
@@ -232,6 +238,12 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     std::memset(&c->march, 0, sizeof(c->march));
     std::memset(&c->tuning, 0, sizeof(c->tuning));
     c->profile = false;
+    c->inst = nullptr;
+    c->inst_rects = nullptr;
+    c->inst_cap = 0;
+    c->tile_lists = nullptr;
+    c->tile_counts = nullptr;
+    c->tile_cap = 0;
     hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
     if (me == hipSuccess) me = hipMalloc(&c->march.counters, kCounterWords * sizeof(uint32_t));
     if (me != hipSuccess) {
@@ -251,6 +263,10 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     (void)hipFree(ctx->shade_items);
     (void)hipFree(ctx->shade_block0);
     march_buffers_free(ctx->march);
+    (void)hipFree(ctx->inst);
+    (void)hipFree(ctx->inst_rects);
+    (void)hipFree(ctx->tile_lists);
+    (void)hipFree(ctx->tile_counts);
     profile_clear(ctx);
     delete ctx;
     return VX_OK;
@@ -837,6 +853,192 @@ extern "C" vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_vie
     A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
     hipLaunchKernelGGL(k_primary, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), A);
     VX_LAUNCHED("k_primary");
+    return VX_OK;
+}
+
+// ---- multi-instance primary: host-side per-draw constants --------------------
+// 4x4 inverse (column-major) in float64 by cofactors and the determinant.
+static bool inverse4(const double* m, double* inv) {
+    double c[16];
+    c[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+           m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    c[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    c[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+           m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    c[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    c[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    c[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+           m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    c[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    c[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    c[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+           m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    c[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+           m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    c[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+            m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    c[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    c[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+           m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    c[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+           m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    c[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+            m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    c[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+            m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    const double det = m[0] * c[0] + m[1] * c[4] + m[2] * c[8] + m[3] * c[12];
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    for (int i = 0; i < 16; ++i) inv[i] = c[i] / det;
+    return true;
+}
+
+// a * b in fp32, column j = ((a0*bj.x + a1*bj.y) + a2*bj.z) + a3*bj.w.
+static void mat_mul_f32(const float* a, const float* b, float* out) {
+    for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 4; ++r)
+            out[j * 4 + r] = ((a[0 + r] * b[j * 4 + 0] + a[4 + r] * b[j * 4 + 1]) + a[8 + r] * b[j * 4 + 2]) +
+                             a[12 + r] * b[j * 4 + 3];
+}
+
+// Conservative pixel rectangle of a draw's box (x0 > x1: nothing visible).
+static void inst_rect(const vx_view& v, const float* world, const int32_t dims[3], int32_t r[4]) {
+    double pv[16];
+    for (int j = 0; j < 4; ++j)
+        for (int q = 0; q < 4; ++q) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; ++k) acc += (double)v.proj[k * 4 + q] * (double)v.view[j * 4 + k];
+            pv[j * 4 + q] = acc;
+        }
+    bool near_or_behind = false, all_behind = true;
+    double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+    for (int c = 0; c < 8; ++c) {
+        const double l[3] = {(c & 1) ? dims[0] * 0.1 : 0.0, (c & 2) ? dims[1] * 0.1 : 0.0,
+                             (c & 4) ? dims[2] * 0.1 : 0.0};
+        double w[4], cl[4];
+        for (int q = 0; q < 4; ++q)
+            w[q] = (double)world[q] * l[0] + (double)world[4 + q] * l[1] + (double)world[8 + q] * l[2] +
+                   (double)world[12 + q];
+        for (int q = 0; q < 4; ++q) cl[q] = pv[q] * w[0] + pv[4 + q] * w[1] + pv[8 + q] * w[2] + pv[12 + q] * w[3];
+        if (cl[3] > -1e-3) all_behind = false;
+        if (cl[3] < 1e-3) {
+            near_or_behind = true;
+            continue;
+        }
+        const double px = (cl[0] / cl[3] + 1.0) * 0.5 * v.res[0] - 0.5 - v.jitter[0];
+        const double py = (1.0 - cl[1] / cl[3]) * 0.5 * v.res[1] - 0.5 - v.jitter[1];
+        x0 = std::min(x0, px); x1 = std::max(x1, px);
+        y0 = std::min(y0, py); y1 = std::max(y1, py);
+    }
+    if (all_behind) {
+        r[0] = 1; r[1] = 1; r[2] = 0; r[3] = 0;
+        return;
+    }
+    if (near_or_behind) {
+        r[0] = 0; r[1] = 0; r[2] = v.width - 1; r[3] = v.height - 1;
+        return;
+    }
+    const double lim = 1e9;
+    x0 = std::max(std::floor(x0) - 2.0, -lim); y0 = std::max(std::floor(y0) - 2.0, -lim);
+    x1 = std::min(std::ceil(x1) + 2.0, lim); y1 = std::min(std::ceil(y1) + 2.0, lim);
+    r[0] = (int32_t)std::max(x0, 0.0); r[1] = (int32_t)std::max(y0, 0.0);
+    r[2] = (int32_t)std::min(x1, (double)(v.width - 1)); r[3] = (int32_t)std::min(y1, (double)(v.height - 1));
+}
+
+extern "C" vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, const vx_view* view,
+                                          const vx_gbuf* g, const vx_gbuf_ext* ext, void* stream) {
+    VX_REQUIRE(ctx && g, "ctx/gbuf is NULL");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g->pos && g->normal && g->material, "gbuf has a NULL pointer");
+    if (n > 4096) return fail(VX_ERR_RANGE, "%u voxel draws, at most 4096", n);
+    VX_REQUIRE(n == 0 || cmds, "cmds is NULL");
+    std::vector<InstConst> h(n);
+    std::vector<int32_t> rects(4 * (size_t)n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const vx_vox_cmd& c = cmds[k];
+        VX_REQUIRE(c.model, "draw %u: model is NULL", k);
+        VX_REQUIRE(c.palette >= 0 && c.palette <= 65535, "draw %u: palette %d outside 0..65535", k, c.palette);
+        InstConst& I = h[k];
+        std::memset(&I, 0, sizeof(I));
+        double wd[16], wi[16];
+        for (int q = 0; q < 16; ++q) {
+            I.w[q] = c.world[q];
+            I.lw[q] = c.last_world[q];
+            wd[q] = c.world[q];
+        }
+        if (!inverse4(wd, wi)) return fail(VX_ERR_INVALID_ARGUMENT, "draw %u: world matrix is singular", k);
+        for (int q = 0; q < 16; ++q) I.winv[q] = (float)wi[q];
+        const double cx = view->cam_pos[0], cy = view->cam_pos[1], cz = view->cam_pos[2];
+        for (int q = 0; q < 3; ++q)
+            I.lc[q] = (float)((((wi[q] * cx + wi[4 + q] * cy) + wi[8 + q] * cz) + wi[12 + q]) * 10.0);
+        I.rid = c.volume_rid;
+        I.palette = c.palette;
+        for (int q = 0; q < 3; ++q) {
+            I.m.mips[q] = c.model->mip[q];
+            I.m.dims[q] = c.model->dims[q];
+        }
+        inst_rect(*view, c.world, c.model->dims, &rects[4 * (size_t)k]);
+    }
+    if (vx_status st = use_device(ctx)) return st;
+    const TileMap tiles = tile_map(view->width, view->height);
+    if (ctx->inst_cap < std::max(n, 1u) || ctx->tile_cap < tiles.ntiles) {
+        VX_HIP(hipDeviceSynchronize());
+        if (ctx->inst_cap < std::max(n, 1u)) {
+            (void)hipFree(ctx->inst);
+            (void)hipFree(ctx->inst_rects);
+            ctx->inst = nullptr;
+            ctx->inst_rects = nullptr;
+            ctx->inst_cap = 0;
+            const uint32_t cap = std::max(n, 64u);
+            VX_HIP(hipMalloc(&ctx->inst, cap * sizeof(InstConst)));
+            VX_HIP(hipMalloc(&ctx->inst_rects, cap * 4 * sizeof(int32_t)));
+            ctx->inst_cap = cap;
+        }
+        if (ctx->tile_cap < tiles.ntiles) {
+            (void)hipFree(ctx->tile_lists);
+            (void)hipFree(ctx->tile_counts);
+            ctx->tile_lists = nullptr;
+            ctx->tile_counts = nullptr;
+            ctx->tile_cap = 0;
+            VX_HIP(hipMalloc(&ctx->tile_lists, (size_t)tiles.ntiles * kTileInstCap * sizeof(uint16_t)));
+            VX_HIP(hipMalloc(&ctx->tile_counts, (size_t)tiles.ntiles * sizeof(uint32_t)));
+            ctx->tile_cap = tiles.ntiles;
+        }
+    }
+    if (n) {
+        VX_HIP(hipMemcpyAsync(ctx->inst, h.data(), n * sizeof(InstConst), hipMemcpyHostToDevice, S(stream)));
+        VX_HIP(hipMemcpyAsync(ctx->inst_rects, rects.data(), rects.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              S(stream)));
+    }
+    BinArgs B;
+    B.rects = ctx->inst_rects;
+    B.ninst = n;
+    B.tiles_x = tiles.tiles_x;
+    B.ntiles = tiles.ntiles;
+    B.lists = ctx->tile_lists;
+    B.counts = ctx->tile_counts;
+    hipLaunchKernelGGL(k_bin_instances, dim3(cdiv(tiles.ntiles, 256)), dim3(256), 0, S(stream), B);
+    VX_LAUNCHED("k_bin_instances");
+    PrimaryInstArgs A;
+    A.view = *view;
+    A.tiles = tiles;
+    mat_mul_f32(view->proj, view->view, A.pv);
+    mat_mul_f32(view->proj, view->last_view, A.plv);
+    A.inst = ctx->inst;
+    A.ninst = n;
+    A.lists = ctx->tile_lists;
+    A.counts = ctx->tile_counts;
+    A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
+    A.motion = ext ? ext->motion : nullptr;
+    A.depth = ext ? ext->depth : nullptr;
+    A.palette = ext ? ext->palette : nullptr;
+    hipLaunchKernelGGL(k_primary_inst, dim3(8 * tiles.per_xcd), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_primary_inst");
     return VX_OK;
 }
 

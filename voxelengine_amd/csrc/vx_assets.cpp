@@ -672,4 +672,40 @@ extern "C" vx_status vx_prefab_lights(const vx_prefab* p, vx_light* out, uint32_
     return VX_OK;
 }
 
+// WorldRenderer.cpp:210-220: the owning group<VoxRenderer>(get<Transform>)
+// walks the VoxRenderer pool from its last element; each draw's matrices are
+// translated by -Pivot (glm::translate, fp32).
+extern "C" vx_status vx_prefab_vox_cmds(const vx_prefab* p, const vx_model* const* models, uint32_t nmodels,
+                                        const float* prev, vx_vox_cmd* out, uint32_t cap, uint32_t* n) {
+    if (!p || !n || (!out && cap) || (!models && nmodels))
+        return failf(VX_ERR_INVALID_ARGUMENT, "vx_prefab_vox_cmds: NULL argument");
+    uint32_t k = 0;
+    for (size_t j = p->vox_pool.size(); j-- > 0;) {
+        const int32_t e = p->vox_pool[j];
+        const vx_prefab_entity& x = p->ents[e];
+        if (!(x.flags & VX_ENT_TRANSFORM) || x.model < 0 || x.palette < 0) continue;
+        if ((uint32_t)x.model >= nmodels || !models[x.model])
+            return failf(VX_ERR_INVALID_ARGUMENT, "model of file %d (%s) is not given", x.model,
+                         p->files[x.model].c_str());
+        if (k < cap) {
+            vx_vox_cmd& c = out[k];
+            memset(&c, 0, sizeof(c));
+            c.model = models[x.model];
+            const float neg[3] = {-x.pivot[0], -x.pivot[1], -x.pivot[2]};
+            memcpy(c.world, x.world, sizeof(c.world));
+            mat_translate(c.world, neg);
+            if (prev) memcpy(c.last_world, prev + 16 * (size_t)e, sizeof(c.last_world));
+            else mat_identity(c.last_world);
+            mat_translate(c.last_world, neg);
+            c.volume_rid = x.model + 1;
+            c.palette = x.palette;
+        }
+        ++k;
+    }
+    *n = k;
+    if (k > cap) return failf(VX_ERR_RANGE, "capacity %u below %u draws", cap, k);
+    return VX_OK;
+}
+
 static_assert(sizeof(vx_prefab_entity) == 256, "vx_prefab_entity is 256 bytes");
+static_assert(sizeof(vx_vox_cmd) == 144, "vx_vox_cmd is 144 bytes");

@@ -81,6 +81,50 @@ struct LightArgs {
     uint32_t* fetches;
 };
 
+// A model's three mips (X-fastest bytes) and mip-0 dims.
+struct ModelMips {
+    const uint8_t* mips[3];
+    int32_t dims[3];
+};
+
+// One VoxCmd with its per-instance constants (host-computed).
+struct InstConst {
+    float w[16];      // WorldMatrix (pivot applied)
+    float lw[16];     // LastWorldMatrix
+    float winv[16];   // inverse(WorldMatrix), from a float64 inverse
+    float lc[3];      // localCameraPos: inverse(WorldMatrix) * camera * 10
+    int32_t rid;      // VolumeRID
+    int32_t palette;  // PalleteIndex
+    int32_t pad;
+    ModelMips m;
+};
+
+constexpr uint32_t kTileInstCap = 64;
+
+struct BinArgs {
+    const int32_t* rects;   // per instance x0, y0, x1, y1 (pixels, inclusive); x0 > x1 = culled
+    uint32_t ninst, tiles_x, ntiles;
+    uint16_t* lists;        // ntiles * kTileInstCap
+    uint32_t* counts;       // ntiles
+};
+
+struct PrimaryInstArgs {
+    vx_view view;
+    TileMap tiles;
+    float pv[16];           // Projection * View
+    float plv[16];          // Projection * LastView
+    const InstConst* inst;
+    uint32_t ninst;
+    const uint16_t* lists;
+    const uint32_t* counts;
+    float* pos;
+    float* nrm;
+    uint8_t* mat;
+    float* motion;
+    float* depth;
+    uint16_t* palette;
+};
+
 struct PrimaryArgs {
     vx_view view;
     TileMap tiles;
@@ -127,6 +171,8 @@ __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A);
 __global__ void k_zero_u32(uint32_t* p, int n);
 __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
+__global__ void __launch_bounds__(256) k_bin_instances(BinArgs A);
+__global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A);
 __global__ void k_shade_fwd(ShadeArgs A);
 __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);
 __global__ void __launch_bounds__(256) k_shade_bwd_batch(const ShadeArgs* items, const uint32_t* block0, uint32_t n,

@@ -1220,15 +1220,16 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
 // Primary hit: GeometryVoxel.frag (LOD 1, MIP_COUNT 2, TRANSPARENCY 1) for the
 // world model with an identity WorldMatrix.
 // ===========================================================================
-VXD uint32_t mip_fetch(const PrimaryArgs& A, int32_t mip, int32_t x, int32_t y, int32_t z) {
+VXD uint32_t mip_fetch(const ModelMips& A, int32_t mip, int32_t x, int32_t y, int32_t z) {
     int32_t mx = A.dims[0] >> mip, my = A.dims[1] >> mip, mz = A.dims[2] >> mip;
     if ((uint32_t)x >= (uint32_t)mx || (uint32_t)y >= (uint32_t)my || (uint32_t)z >= (uint32_t)mz) return 0u;
     return A.mips[mip][(size_t)x + (size_t)y * mx + (size_t)z * mx * my];
 }
 
-// intersectVolume, GeometryVoxel.frag:63-125.
-VXD int32_t intersect_volume(const PrimaryArgs& A, f3 o3, f3 d3, f3 cam, float uvx, float uvy, f3& hit,
-                             f3& nrm, uint32_t& material) {
+// intersectVolume, GeometryVoxel.frag:63-125, in the model's voxel space.
+// res / frame: the view's resolution and frame (the glass dither).
+VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t frame, f3 o3, f3 d3, f3 cam,
+                             float uvx, float uvy, f3& hit, f3& nrm, uint32_t& material) {
     float o[3] = {o3.x, o3.y, o3.z}, d[3] = {d3.x, d3.y, d3.z}, c[3] = {cam.x, cam.y, cam.z};
     float s[3], td[3];
 #pragma unroll
@@ -1270,9 +1271,9 @@ VXD int32_t intersect_volume(const PrimaryArgs& A, f3 o3, f3 d3, f3 cam, float u
                     accept = (float)mip < 0.001f * sqrtf(dot3(df, df));
                 }
                 if (accept) {
-                    float cx = roundf((uvx * A.view.res[0]) * 0.5f);
-                    float cy = roundf((uvy * A.view.res[1]) * 0.5f);
-                    bool glass = vox < 16u && gmod(cy + cx, 2.0f) == (float)(A.view.frame % 2);
+                    float cx = roundf((uvx * res[0]) * 0.5f);
+                    float cy = roundf((uvy * res[1]) * 0.5f);
+                    bool glass = vox < 16u && gmod(cy + cx, 2.0f) == (float)(frame % 2);
                     if (!glass) {
                         nrm = mk3(-s[0] * s0, -s[1] * s1, -s[2] * s2);
                         hit = mk3((o[0] + d[0] * best_t) * mip_size, (o[1] + d[1] * best_t) * mip_size,
@@ -1297,43 +1298,55 @@ VXD int32_t intersect_volume(const PrimaryArgs& A, f3 o3, f3 d3, f3 cam, float u
     return 0;
 }
 
+// Pinhole ray of pixel (px, py) from the inverse projection / view
+// (LightAmbient.vert:32-36), normalized; ndc: the pixel's normalized coordinates.
+VXD f3 pixel_dir(const vx_view& vw, int32_t px, int32_t py, float& ndc_x, float& ndc_y) {
+    float u = ((float)px + 0.5f + vw.jitter[0]) / vw.res[0];
+    float v = ((float)py + 0.5f + vw.jitter[1]) / vw.res[1];
+    ndc_x = u * 2.0f - 1.0f;
+    ndc_y = 1.0f - v * 2.0f;
+    float c4[4] = {ndc_x, ndc_y, 1.0f, 1.0f}, f4[4], w4[4];
+    mat_vec(vw.inv_proj, c4, f4);
+    float fv[4] = {f4[0] / f4[3], f4[1] / f4[3], f4[2] / f4[3], 0.0f};
+    mat_vec(vw.inv_view, fv, w4);
+    return normalize3(mk3(w4[0], w4[1], w4[2]));
+}
+
+// clipToAABB, GeometryVoxel.frag:49-60.
+VXD f3 clip_to_aabb(f3 o, f3 d, const int32_t dims[3]) {
+    float size[3] = {(float)dims[0], (float)dims[1], (float)dims[2]};
+    float co[3] = {o.x, o.y, o.z}, cd[3] = {d.x, d.y, d.z};
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (gclamp(co[k], 0.0f, size[k]) != co[k]) inside = false;
+    if (inside) return o;
+    float t[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float inv = 1.0f / cd[k];
+        float sgn = gstep(cd[k], 0.0f);
+        t[k] = (sgn * size[k] - co[k]) * inv;
+    }
+    float tmin = gmax(gmax(t[0], t[1]), t[2]);
+    return mk3(co[0] + cd[0] * (tmin - 0.001f), co[1] + cd[1] * (tmin - 0.001f), co[2] + cd[2] * (tmin - 0.001f));
+}
+
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A) {
     int32_t px, py;
     if (!tile_pixel(A.tiles, px, py)) return;
     const vx_view& vw = A.view;
     size_t i = (size_t)py * vw.width + px;
-    // Pinhole ray from the inverse projection / view (LightAmbient.vert:32-36).
-    float u = ((float)px + 0.5f + vw.jitter[0]) / vw.res[0];
-    float v = ((float)py + 0.5f + vw.jitter[1]) / vw.res[1];
-    float ndc_x = u * 2.0f - 1.0f, ndc_y = 1.0f - v * 2.0f;
-    float c4[4] = {ndc_x, ndc_y, 1.0f, 1.0f}, f4[4], w4[4];
-    mat_vec(vw.inv_proj, c4, f4);
-    float fv[4] = {f4[0] / f4[3], f4[1] / f4[3], f4[2] / f4[3], 0.0f};
-    mat_vec(vw.inv_view, fv, w4);
-    f3 dir = normalize3(mk3(w4[0], w4[1], w4[2]));
+    float ndc_x, ndc_y;
+    f3 dir = pixel_dir(vw, px, py, ndc_x, ndc_y);
     f3 cam = mk3(vw.cam_pos[0] * 10.0f, vw.cam_pos[1] * 10.0f, vw.cam_pos[2] * 10.0f);
-    // clipToAABB, GeometryVoxel.frag:49-60.
-    float size[3] = {(float)A.dims[0], (float)A.dims[1], (float)A.dims[2]};
-    float co[3] = {cam.x, cam.y, cam.z}, cd[3] = {dir.x, dir.y, dir.z};
-    bool inside = true;
+    ModelMips M;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (gclamp(co[k], 0.0f, size[k]) != co[k]) inside = false;
-    f3 start = cam;
-    if (!inside) {
-        float t[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float inv = 1.0f / cd[k];
-            float sgn = gstep(cd[k], 0.0f);
-            t[k] = (sgn * size[k] - co[k]) * inv;
-        }
-        float tmin = gmax(gmax(t[0], t[1]), t[2]);
-        start = mk3(co[0] + cd[0] * (tmin - 0.001f), co[1] + cd[1] * (tmin - 0.001f), co[2] + cd[2] * (tmin - 0.001f));
-    }
+    for (int k = 0; k < 3; ++k) { M.mips[k] = A.mips[k]; M.dims[k] = A.dims[k]; }
+    f3 start = clip_to_aabb(cam, dir, M.dims);
     f3 hit = mk3(0.0f, 0.0f, 0.0f), nrm = mk3(0.0f, 0.0f, 0.0f);
     uint32_t mat = 0;
-    int32_t h = intersect_volume(A, start, dir, cam, ndc_x, ndc_y, hit, nrm, mat);
+    int32_t h = intersect_volume(M, vw.res, vw.frame, start, dir, cam, ndc_x, ndc_y, hit, nrm, mat);
     if (!h) {
         hit = mk3(0.0f, 0.0f, 0.0f);
         nrm = mk3(0.0f, 0.0f, 0.0f);
@@ -1342,6 +1355,114 @@ __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A) {
     A.pos[3 * i + 0] = hit.x; A.pos[3 * i + 1] = hit.y; A.pos[3 * i + 2] = hit.z;
     A.nrm[3 * i + 0] = nrm.x; A.nrm[3 * i + 1] = nrm.y; A.nrm[3 * i + 2] = nrm.z;
     A.mat[i] = (uint8_t)mat;
+}
+
+// ===========================================================================
+// Multi-instance primary hit: GeometryVoxelPipeline::Use (GeometryVoxelPipeline.h:48-70)
+// draws each VoxCmd's box, GeometryVoxel.vert:62-78 / GeometryVoxel.frag:127-183
+// march it in model space and the D24 depth test (LESS) keeps the nearest
+// fragment. Here each pixel's camera ray is taken into every instance binned
+// to its 16x16 tile, in draw order, and the same depth test picks the result.
+// ===========================================================================
+// One thread per tile: the instances whose conservative screen rectangle
+// covers the tile, in draw order; more than kTileInstCap marks the tile as
+// "every instance".
+__global__ void __launch_bounds__(256) k_bin_instances(BinArgs A) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.ntiles) return;
+    int32_t x0 = (int32_t)(t % A.tiles_x) * 16, y0 = (int32_t)(t / A.tiles_x) * 16;
+    int32_t x1 = x0 + 15, y1 = y0 + 15;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < A.ninst; ++k) {
+        const int32_t* r = A.rects + 4 * k;
+        if (r[0] > x1 || r[2] < x0 || r[1] > y1 || r[3] < y0) continue;
+        if (c < kTileInstCap) A.lists[(size_t)t * kTileInstCap + c] = (uint16_t)k;
+        ++c;
+    }
+    A.counts[t] = c;
+}
+
+// D24 UNORM conversion of a fragment depth (clamped to the depth range).
+VXD uint32_t depth_d24(float d) {
+    d = gclamp(d, 0.0f, 1.0f);
+    return (uint32_t)rintf(d * 16777215.0f);
+}
+
+__global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A) {
+    __shared__ uint16_t list[kTileInstCap];
+    __shared__ uint32_t count;
+    uint32_t b = blockIdx.x;
+    uint32_t logical = (b % 8u) * A.tiles.per_xcd + b / 8u;
+    if (logical >= A.tiles.ntiles) return;   // uniform over the block
+    if (threadIdx.x == 0) count = A.counts[logical];
+    if (threadIdx.x < kTileInstCap) list[threadIdx.x] = A.lists[(size_t)logical * kTileInstCap + threadIdx.x];
+    __syncthreads();
+    int32_t px, py;
+    if (!tile_pixel(A.tiles, px, py)) return;
+    const vx_view& vw = A.view;
+    const size_t i = (size_t)py * vw.width + px;
+    float ndc_x, ndc_y;
+    const f3 dw = pixel_dir(vw, px, py, ndc_x, ndc_y);
+    const uint32_t c = count;
+    const uint32_t n = c > kTileInstCap ? A.ninst : c;
+    uint32_t best_q = 16777215u;      // cleared depth 1.0
+    int32_t best = -1;
+    f3 bhit = mk3(0.0f, 0.0f, 0.0f), bn = mk3(0.0f, 0.0f, 0.0f);
+    uint32_t bmat = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t k = c > kTileInstCap ? j : (uint32_t)list[j];
+        const InstConst& I = A.inst[k];
+        f3 dl = mk3((I.winv[0] * dw.x + I.winv[4] * dw.y) + I.winv[8] * dw.z,
+                    (I.winv[1] * dw.x + I.winv[5] * dw.y) + I.winv[9] * dw.z,
+                    (I.winv[2] * dw.x + I.winv[6] * dw.y) + I.winv[10] * dw.z);
+        dl = normalize3(dl);
+        const f3 lc = mk3(I.lc[0], I.lc[1], I.lc[2]);
+        const f3 start = clip_to_aabb(lc, dl, I.m.dims);
+        f3 hit, nrm;
+        uint32_t mat = 0;
+        if (!intersect_volume(I.m, vw.res, vw.frame, start, dl, lc, ndc_x, ndc_y, hit, nrm, mat)) continue;
+        float h4[4] = {hit.x * 0.1f, hit.y * 0.1f, hit.z * 0.1f, 1.0f}, wh[4], cur[4];
+        mat_vec(I.w, h4, wh);
+        mat_vec(A.pv, wh, cur);
+        const float depth = ((1.0f - 1e-7f * (float)I.rid) * (cur[3] - 0.1f)) / (4096.0f - 0.1f);
+        const uint32_t q = depth_d24(depth);
+        if (q < best_q) {
+            best_q = q;
+            best = (int32_t)k;
+            bhit = hit;
+            bn = nrm;
+            bmat = mat;
+        }
+    }
+    float pos[3] = {0.0f, 0.0f, 0.0f}, nw[3] = {0.0f, 0.0f, 0.0f}, mo[2] = {0.0f, 0.0f};
+    uint16_t pal = 0;
+    if (best >= 0) {
+        const InstConst& I = A.inst[best];
+        float h4[4] = {bhit.x * 0.1f, bhit.y * 0.1f, bhit.z * 0.1f, 1.0f}, wh[4], cur[4], lw[4], last[4];
+        mat_vec(I.w, h4, wh);
+        mat_vec(A.pv, wh, cur);
+        mat_vec(I.lw, h4, lw);
+        mat_vec(A.plv, lw, last);
+        // out_Normal = normalize(WorldMatrix * vec4(normal, 0))
+        float n4[4] = {bn.x, bn.y, bn.z, 0.0f}, wn[4];
+        mat_vec(I.w, n4, wn);
+        const float l = sqrtf(((wn[0] * wn[0] + wn[1] * wn[1]) + wn[2] * wn[2]) + wn[3] * wn[3]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            pos[k] = wh[k] * 10.0f;
+            nw[k] = wn[k] / l;
+        }
+        // out_Motion = -vec2(0.5, -0.5) * (cur.xy / cur.w - last.xy / last.w)
+        mo[0] = -0.5f * (cur[0] / cur[3] - last[0] / last[3]);
+        mo[1] = 0.5f * (cur[1] / cur[3] - last[1] / last[3]);
+        pal = (uint16_t)I.palette;
+    }
+    A.pos[3 * i + 0] = pos[0]; A.pos[3 * i + 1] = pos[1]; A.pos[3 * i + 2] = pos[2];
+    A.nrm[3 * i + 0] = nw[0]; A.nrm[3 * i + 1] = nw[1]; A.nrm[3 * i + 2] = nw[2];
+    A.mat[i] = (uint8_t)bmat;
+    if (A.motion) { A.motion[2 * i + 0] = mo[0]; A.motion[2 * i + 1] = mo[1]; }
+    if (A.depth) A.depth[i] = (float)best_q / 16777215.0f;
+    if (A.palette) A.palette[i] = pal;
 }
 
 // ===========================================================================
