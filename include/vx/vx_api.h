@@ -249,7 +249,10 @@ typedef struct {
  * GeometryVoxelPipeline.h:48-70; GeometryVoxel.vert:62-78,
  * GeometryVoxel.frag:127-183): each pixel's camera ray is marched through
  * every draw whose box covers it, in model space with the model's mips, and
- * the D24 depth test (LESS, draws in array order) keeps the nearest hit.
+ * the D24 depth test (LESS, draws in array order) keeps the nearest hit. A box
+ * covers a pixel when the pixel's forward ray leaves it at a view depth >= NEAR
+ * (its inward-facing triangles and back-face culling draw the far faces,
+ * GeometryVoxel.vert:5-42, evk.cpp:470-471).
  * gbuf.pos is the world hit point in fine-voxel units (world * 10), gbuf.normal
  * the world normal. n <= 4096 (MAX_INSTANCES, GeometryVoxelPipeline.h:17). */
 vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, const vx_view* view,

@@ -580,6 +580,25 @@ static void mat_mul_f(const float* a, const float* b, float* out) {
                              a[12 + r] * b[j * 4 + 3];
 }
 
+/* The draw's box rasterizes at the pixel when the forward ray leaves it at a
+ * view depth >= NEAR: the CUBE triangles (GeometryVoxel.vert:5-42) face
+ * inward and back faces are culled with CCW fronts (evk.cpp:470-471), so the
+ * far faces are the ones drawn, clipped by the near plane. */
+static int box_covers(const vxo_vox_cmd* c, const float o[3], const float d[3], const float* pv) {
+    float tn = -INFINITY, tf = INFINITY;
+    for (int k = 0; k < 3; ++k) {
+        float inv = 1.0f / d[k];
+        float t0 = (0.0f - o[k]) * inv, t1 = ((float)c->dims[k] - o[k]) * inv;
+        tn = fmaxf(tn, fminf(t0, t1));
+        tf = fminf(tf, fmaxf(t0, t1));
+    }
+    if (!(tn <= tf && tf > 0.0f)) return 0;
+    float pe[4] = {(o[0] + d[0] * tf) * 0.1f, (o[1] + d[1] * tf) * 0.1f, (o[2] + d[2] * tf) * 0.1f, 1.0f}, we[4], ce[4];
+    mat_vec(c->world, pe, we);
+    mat_vec(pv, we, ce);
+    return ce[3] >= 0.1f;
+}
+
 /* The geometry pass per pixel: GeometryVoxel.vert:72-75 gives the camera in
  * model space (inverse(WorldMatrix) * camera * 10) and the direction toward
  * the fragment; GeometryVoxel.frag:131-146 clips to the box and marches; the
@@ -623,6 +642,7 @@ void vxo_primary_instances(const vxo_vox_cmd* cmds, int32_t n, const vxo_view* v
                 dr[1] = (Wi[1] * dw[0] + Wi[5] * dw[1]) + Wi[9] * dw[2];
                 dr[2] = (Wi[2] * dw[0] + Wi[6] * dw[1]) + Wi[10] * dw[2];
                 normalize3(dr, dl);
+                if (!box_covers(&cmds[k], lc[k], dl, pv)) continue;
                 float size[3] = {(float)cmds[k].dims[0], (float)cmds[k].dims[1], (float)cmds[k].dims[2]};
                 vxo_clip_to_aabb(lc[k], dl, size, start);
                 uint8_t m = 0;

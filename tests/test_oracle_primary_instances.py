@@ -159,3 +159,23 @@ def test_scaled_rotated_instances_hit_their_boxes(oracle):
         ax = (np.linalg.inv(Mw[:3, :3]) @ n.T).T            # back to model axes: one non-zero component
         ax /= np.linalg.norm(ax, axis=1, keepdims=True)
         assert np.all(np.sort(np.abs(ax), axis=1)[:, 1] < 1e-5)
+
+
+def test_draws_behind_the_camera_cover_nothing_and_a_box_around_it_is_drawn(oracle):
+    """The far faces are the rasterized ones: a box behind the camera covers
+    no pixel (its voxels would otherwise be hit by the back-projected
+    clipToAABB start), and a box around the camera covers every pixel."""
+    vol = _model(8)
+    vd = views.make_view((-0.5, 4.0, -0.5), (2.5, 0.5, 2.5), W, H)
+    front = _cmd(oracle, vol, _translate((0.5, 0.0, 0.5)), palette=1)
+    behind = _cmd(oracle, vol, _translate((-9.0, 3.0, -9.0)), palette=2)
+    r1 = oracle.primary_instances([front], vd)
+    r2 = oracle.primary_instances([behind, front], vd)
+    for k in r1:
+        np.testing.assert_array_equal(r1[k], r2[k])
+    around = _cmd(oracle, np.full((16, 16, 16), 0, np.uint8), _translate((-1.3, 3.2, -1.3)))
+    around["mips"][0][:, :, 0] = 33                       # the x = 0 wall, behind the camera
+    around["mips"][0][:, :, 15] = 44                      # the x = 15 wall, ahead of it
+    around["mips"] = oracle.build_mips(around["mips"][0])
+    r = oracle.primary_instances([around], vd)
+    assert (r["mat"] == 44).mean() > 0.2 and not (r["mat"] == 33).any()

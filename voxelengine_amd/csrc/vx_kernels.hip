@@ -1382,6 +1382,28 @@ __global__ void __launch_bounds__(256) k_bin_instances(BinArgs A) {
     A.counts[t] = c;
 }
 
+// Whether the draw's box rasterizes at this pixel. The CUBE triangles
+// (GeometryVoxel.vert:5-42) face inward and the pipeline culls back faces
+// with CCW fronts (evk.cpp:470-471), so the far faces are drawn: the pixel is
+// covered when its forward ray leaves the box at a view depth >= NEAR (the
+// near-plane clip). Slabs in model voxel units from the local camera.
+VXD bool box_covers(const InstConst& I, f3 lc, f3 dl, const float* pv) {
+    const float o[3] = {lc.x, lc.y, lc.z}, d[3] = {dl.x, dl.y, dl.z};
+    float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float inv = 1.0f / d[k];
+        const float t0 = (0.0f - o[k]) * inv, t1 = ((float)I.m.dims[k] - o[k]) * inv;
+        tn = fmaxf(tn, fminf(t0, t1));
+        tf = fminf(tf, fmaxf(t0, t1));
+    }
+    if (!(tn <= tf && tf > 0.0f)) return false;
+    float pe[4] = {(o[0] + d[0] * tf) * 0.1f, (o[1] + d[1] * tf) * 0.1f, (o[2] + d[2] * tf) * 0.1f, 1.0f}, we[4], ce[4];
+    mat_vec(I.w, pe, we);
+    mat_vec(pv, we, ce);
+    return ce[3] >= 0.1f;
+}
+
 // D24 UNORM conversion of a fragment depth (clamped to the depth range).
 VXD uint32_t depth_d24(float d) {
     d = gclamp(d, 0.0f, 1.0f);
@@ -1417,6 +1439,7 @@ __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A) {
                     (I.winv[2] * dw.x + I.winv[6] * dw.y) + I.winv[10] * dw.z);
         dl = normalize3(dl);
         const f3 lc = mk3(I.lc[0], I.lc[1], I.lc[2]);
+        if (!box_covers(I, lc, dl, A.pv)) continue;
         const f3 start = clip_to_aabb(lc, dl, I.m.dims);
         f3 hit, nrm;
         uint32_t mat = 0;
