@@ -11,11 +11,12 @@ from voxelengine_amd import views
 pytestmark = pytest.mark.gpu
 
 
-def _model(seed, dims, fill=0.08):
+def _model(seed, dims, fill=0.08, floor=True):
     rng = np.random.default_rng(seed)
     sx, sy, sz = dims
     m = (rng.random((sz, sy, sx)) < fill).astype(np.uint8) * rng.integers(1, 255, (sz, sy, sx), dtype=np.uint8)
-    m[:, :2, :] = rng.integers(1, 255, dtype=np.uint8)      # a floor in y
+    if floor:
+        m[:, :2, :] = rng.integers(1, 255, dtype=np.uint8)      # a floor in y
     return m
 
 
@@ -65,7 +66,8 @@ def _run(ctx, oracle, models_np, draws, vd):
 
 def test_random_draws_match_oracle(ctx, oracle):
     rng = np.random.default_rng(11)
-    models = [_model(1, (32, 32, 32)), _model(2, (16, 24, 40)), _model(3, (48, 8, 20), fill=0.2)]
+    models = [_model(1, (32, 32, 32)), _model(2, (16, 24, 40)), _model(3, (48, 8, 20), fill=0.2),
+              _model(9, (32, 32, 32), fill=0.002, floor=False)]
     draws = []
     for k in range(10):
         mi = k % 3
@@ -73,7 +75,7 @@ def test_random_draws_match_oracle(ctx, oracle):
         lw = _world(_rot(rng), rng.uniform(0.5, 1.6), rng.uniform(-3.0, 6.0, 3))
         draws.append((mi, w, lw, int(rng.integers(0, 5000)), k + 1))
     # one draw around the camera, one behind it
-    draws.append((0, _world(np.eye(3), 1.0, (-1.6, 3.2, -1.6)), None, 7, 20))
+    draws.append((3, _world(np.eye(3), 1.0, (-1.6, 3.2, -1.6)), None, 7, 20))
     draws.append((1, _world(np.eye(3), 1.0, (-9.0, 6.0, -9.0)), None, 8, 21))
     draws = [(k, w, w if lw is None else lw, rid, p) for (k, w, lw, rid, p) in draws]
     vd = views.make_view((-0.5, 4.0, -0.5), (2.5, 0.5, 2.5), 160, 120, frame=3, jitter=(0.25, -0.375))
