@@ -183,6 +183,45 @@ def bench_large(ctx, pal_d, reps=3):
     return out
 
 
+def _prefab_frame(ctx, P, models, vol, reps=10):
+    """One 1920x1080 frame of a loaded prefab: the multi-instance geometry pass
+    over its draws, then 1 shadow + 16 AO + 1 spec DDA rays per hit pixel over
+    that G-buffer against the prefab's rebuilt volume."""
+    import torch
+    from voxelengine_amd import api, configs, views
+    W, H = 1920, 1080
+    vd = views.make_view((8.0, 16.0, 6.0), (32.0, 0.0, 32.0), W, H)
+    cmds = P.vox_cmds(models)
+    g = api.GBuffer(W, H)
+    ext = api.GBufferExt(W, H)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    api.primary_instances(ctx, cmds, vd, g, ext)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        api.primary_instances(ctx, cmds, vd, g, ext)
+    e1.record()
+    torch.cuda.synchronize()
+    geo_ms = e0.elapsed_time(e1) / reps
+    vol.build_levels()
+    hit = int((g.material > 0).sum().item())
+    params = configs.trace_params(**configs.CONFIGS["F"]["params"])
+    rays = hit * configs.rays_per_pixel(params)
+    pal_d = torch.from_numpy(configs.make_palette()).cuda()
+    vis = api.Visibility(W, H)
+    api.trace(ctx, vol, vd, g, pal_d, params, vis)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        api.trace(ctx, vol, vd, g, pal_d, params, vis)
+    e1.record()
+    torch.cuda.synchronize()
+    tr_ms = e0.elapsed_time(e1) / reps
+    return {"draws": len(cmds), "hit_pixels": hit, "geometry_ms": round(geo_ms, 3),
+            "geometry_mpx_per_s": round(W * H / (geo_ms * 1e-3) / 1e6, 1), "trace_ms": round(tr_ms, 3),
+            "trace_grays_per_s": round(rays / (tr_ms * 1e-3) / 1e9, 3)}
+
+
 def bench_prefabs(ctx, reps=5):
     """The reference's FarmHouse and ModernHouse prefabs (files rebuilt from
     tests/golden/prefabs.npz) through the loaders into the 524x188x524 volume:
@@ -221,6 +260,8 @@ def bench_prefabs(ctx, reps=5):
             res[name] = {"entities": len(P.entities), "instances": len(insts), "model_voxels": voxels,
                          "spawn_ms": round(spawn_ms, 3), "model_load_ms": round(load_ms, 2),
                          "rebuild_ms": round(min(times), 3)}
+            if name == "FarmHouse":
+                res[name]["frame_1080p"] = _prefab_frame(ctx, P, models, vol)
             vol.close()
             for m in models.values():
                 m.close()

@@ -121,6 +121,16 @@ public:
         idx.resize(n);
         return idx;
     }
+    // The geometry pass's draws (WorldRenderer.cpp:210-220); models[i]: loaded
+    // model of file i or nullptr; prev: per-entity PreviousWorldMatrix or nullptr.
+    std::vector<vx_vox_cmd> VoxCmds(const std::vector<const vx_model*>& models, const float* prev = nullptr) const {
+        uint32_t n = 0;
+        const vx_status s = vx_prefab_vox_cmds(h_, models.data(), (uint32_t)models.size(), prev, nullptr, 0, &n);
+        if (s != VX_OK && s != VX_ERR_RANGE) check(s);
+        std::vector<vx_vox_cmd> out(n);
+        if (n) check(vx_prefab_vox_cmds(h_, models.data(), (uint32_t)models.size(), prev, out.data(), n, &n));
+        return out;
+    }
     // Point lights, then spot lights, in the reference's draw order.
     std::vector<vx_light> Lights() const {
         uint32_t n = 0;
@@ -267,6 +277,14 @@ public:
     }
     void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const VoxAsset& world, const vx_gbuf& geometryFB) {
         check(vx_primary(ctx.get(), world.GetImage(), &viewBuffer, &geometryFB, cmd));
+    }
+    // Use(CmdBuffer&, Buffer& viewBuffer, glm::vec2 res, Image& blueNoise, std::vector<Cmd>& cmds)
+    // (GeometryVoxelPipeline.h:48-70): every voxel draw, depth-tested in order.
+    using Cmd = vx_vox_cmd;
+    static constexpr int MAX_INSTANCES = 4096;
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const std::vector<Cmd>& cmds, const vx_gbuf& geometryFB,
+             const vx_gbuf_ext* extra = nullptr) {
+        check(vx_primary_instances(ctx.get(), cmds.data(), (uint32_t)cmds.size(), &viewBuffer, &geometryFB, extra, cmd));
     }
 };
 

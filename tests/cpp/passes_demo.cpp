@@ -25,6 +25,10 @@ static int spawn_prefab(vx::Context& ctx, const char* mods, const char* prefab, 
     }
     vx::ShadowVoxSystem shadowVox(ctx);
     const std::vector<vx::Entity> ids = shadowVox.Spawn(pf, models);
+    // the geometry pass's draw list for the same spawn
+    std::vector<const vx_model*> byFile(pf.Files().size(), nullptr);
+    for (const auto& kv : owned) byFile[kv.first] = kv.second->GetImage();
+    const std::vector<vx_vox_cmd> draws = pf.VoxCmds(byFile);
     shadowVox.OnUpdate(0.016f);
     int32_t d[3], levels = 0;
     vx::check(vx_volume_dims(shadowVox.GetVolumeImage(), d, &levels));
@@ -37,8 +41,9 @@ static int spawn_prefab(vx::Context& ctx, const char* mods, const char* prefab, 
         return 1;
     }
     std::fclose(f);
-    std::printf("spawned %zu entities, %zu instances, %zu lights, %zu dirty regions\n", pf.Entities().size(),
-                ids.size(), pf.Lights().size(), shadowVox.LastDirtyRegions().size());
+    std::printf("spawned %zu entities, %zu instances, %zu lights, %zu draws, %zu dirty regions\n",
+                pf.Entities().size(), ids.size(), pf.Lights().size(), draws.size(),
+                shadowVox.LastDirtyRegions().size());
     return 0;
 }
 

@@ -93,5 +93,5 @@ def test_cpp_shadowvox_spawn_matches_reference_order(mods, z, tmp_path):
     out = tmp_path / "farmhouse.bin"
     r = subprocess.run([demo, mods, "default/FarmHouse.pf", str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "67 instances, 1 lights" in r.stdout, r.stdout
+    assert "67 instances, 1 lights, 67 draws" in r.stdout, r.stdout
     assert hashlib.sha256(out.read_bytes()).hexdigest() == z["FarmHouse_ref_frame0_sha256"].tobytes().hex()
