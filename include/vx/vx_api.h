@@ -338,6 +338,26 @@ vx_status vx_light_pass(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, 
                         const float* palette_dev, const vx_light* lights_dev, uint32_t nlights,
                         const vx_light_params* params, float* rgb_dev, uint32_t* fetches_dev, void* stream);
 
+/* LightTAA inputs and output (device; width*height pixels, rows from the top). */
+typedef struct {
+    const float* light;        /* 3 per pixel: this frame's light buffer (ambient + lights) */
+    const float* last;         /* 4 per pixel: last frame's TAA output (rgb, variance) */
+    const float* motion;       /* 2 per pixel (vx_gbuf_ext.motion); NULL = no motion */
+    const float* depth;        /* 1 per pixel (vx_gbuf_ext.depth, 1 = sky) */
+    const float* normal;       /* 3 per pixel (vx_gbuf.normal) */
+    const uint8_t* material;   /* vx_gbuf.material: colour and material from the palette */
+    const float* palette;      /* 256 x 6 */
+    float* out;                /* 4 per pixel: rgb, variance; must not alias last */
+} vx_taa_io;
+
+/* Temporal accumulation of the light buffer (LightTAAPipeline::Use,
+ * LightTAAPipeline.h:35-51; LightTAA.frag:37-141): a 12-sample spiral of
+ * same-surface neighbours, the last frame's light (at uv + motion) clipped to
+ * the neighbourhood's range and blended by the running variance; pixels whose
+ * last position left the screen average a 19x19 same-surface neighbourhood.
+ * Uses view width/height and frame; seed keys the per-pixel spiral angle. */
+vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa_io* io, uint32_t seed, void* stream);
+
 /* One view of a batched adjoint: its G-buffer, fixed visibility, optional
  * SSAO and the loss gradient w.r.t. its shaded image (all device). */
 typedef struct {

@@ -312,6 +312,30 @@ def primary_instances(cmds, vd: dict, nthreads=0):
     return dict(pos=pos, nrm=nrm, mat=mat, motion=motion, depth=depth, palette=pal)
 
 
+def light_taa(width, height, frame, seed, light, last, motion, depth, normal, mat, palette, nthreads=0):
+    """LightTAA over numpy buffers; returns (npx, 4) float32."""
+    npx = width * height
+    f = lambda a, n: np.ascontiguousarray(np.asarray(a, np.float32).reshape(npx, n))
+    light, last, depth, normal = f(light, 3), f(last, 4), f(depth, 1), f(normal, 3)
+    motion = None if motion is None else f(motion, 2)
+    mat = np.ascontiguousarray(np.asarray(mat, np.uint8).reshape(npx))
+    palette = np.ascontiguousarray(np.asarray(palette, np.float32).reshape(256, 6))
+    out = np.zeros((npx, 4), np.float32)
+    lib().vxo_light_taa(C.c_int32(width), C.c_int32(height), C.c_int32(frame), C.c_uint32(seed & 0xFFFFFFFF),
+                        _p(light), _p(last), _p(motion) if motion is not None else None, _p(depth), _p(normal),
+                        _p(mat), _p(palette), _p(out), C.c_int32(nthreads))
+    return out
+
+
+def taa_sample(img, u, v):
+    """The TAA sampler over an (h, w, n) float image at (u, v)."""
+    img = np.ascontiguousarray(img, np.float32)
+    h, w, n = img.shape
+    out = (C.c_float * n)()
+    lib().vxo_taa_sample(_p(img), C.c_int32(w), C.c_int32(h), C.c_int32(n), C.c_float(u), C.c_float(v), out)
+    return np.array(out[:], np.float32)
+
+
 def trace_params(p: dict) -> TraceParams:
     t = TraceParams()
     t.shadow_march = p["shadow_march"]; t.ao_march = p["ao_march"]; t.spec_march = p["spec_march"]

@@ -688,6 +688,177 @@ void vxo_primary_instances(const vxo_vox_cmd* cmds, int32_t n, const vxo_view* v
     (void)nthreads;
 }
 
+/* ---- LightTAA (next rank 4) ---- */
+/* A TAA input: per-pixel attribute with the border of an opaque-black
+ * clamp-to-border sampler (rgb 0, alpha 1). kind selects how texels are made. */
+typedef struct {
+    int32_t w, h, kind, n;   /* kind 0: plain image of n channels; 1: light (+alpha from mat);
+                                2: colour from palette; 3: material from palette */
+    const float* img;
+    const uint8_t* mat;
+    const float* pal;
+} taa_tex;
+
+static float taa_texel(const taa_tex* t, int32_t x, int32_t y, int c) {
+    if (x < 0 || y < 0 || x >= t->w || y >= t->h) return c == 3 ? 1.0f : 0.0f;
+    size_t i = (size_t)y * t->w + x;
+    switch (t->kind) {
+        case 1: return c < 3 ? t->img[3 * i + c] : (t->mat[i] == 0 ? 1.0f : 0.0f);
+        case 2: return t->mat[i] ? t->pal[6 * t->mat[i] + c] : 0.0f;
+        case 3: return (t->mat[i] && c < 3) ? t->pal[6 * t->mat[i] + 3 + c] : 0.0f;
+        default: return t->img ? t->img[(size_t)t->n * i + c] : 0.0f;
+    }
+}
+
+/* Position x in texels to the left texel and the 8-bit weight of its right neighbour. */
+static void taa_weight(float x, int32_t* x0, float* f) {
+    float fl = floorf(x);
+    float w = floorf((x - fl) * 256.0f + 0.5f) * (1.0f / 256.0f);
+    if (w >= 1.0f) { fl = fl + 1.0f; w = 0.0f; }
+    *x0 = (int32_t)fl;
+    *f = w;
+}
+
+static void taa_sample(const taa_tex* t, float u, float v, int n, float* out) {
+    int32_t x0, y0;
+    float fx, fy;
+    taa_weight(u * (float)t->w - 0.5f, &x0, &fx);
+    taa_weight(v * (float)t->h - 0.5f, &y0, &fy);
+    for (int c = 0; c < n; ++c) {
+        float a = taa_texel(t, x0, y0, c), b = taa_texel(t, x0 + 1, y0, c);
+        float cc = taa_texel(t, x0, y0 + 1, c), d = taa_texel(t, x0 + 1, y0 + 1, c);
+        out[c] = (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (cc * (1.0f - fx) + d * fx) * fy;
+    }
+}
+
+void vxo_taa_sample(const float* img, int32_t width, int32_t height, int32_t n, float u, float v, float* out) {
+    taa_tex t = {width, height, 0, n, img, NULL, NULL};
+    taa_sample(&t, u, v, n, out);
+}
+
+static float taa_len(const float* d, int n) {
+    float s = d[0] * d[0];
+    for (int k = 1; k < n; ++k) s = s + d[k] * d[k];
+    return sqrtf(s);
+}
+
+void vxo_light_taa(int32_t W, int32_t H, int32_t frame, uint32_t seed, const float* light, const float* last,
+                   const float* motion, const float* depth, const float* normal, const uint8_t* mat,
+                   const float* palette, float* out, int32_t nthreads) {
+    const taa_tex T_light = {W, H, 1, 3, light, mat, palette}, T_last = {W, H, 0, 4, last, mat, palette};
+    const taa_tex T_motion = {W, H, 0, 2, motion, mat, palette}, T_color = {W, H, 2, 3, NULL, mat, palette};
+    const taa_tex T_matl = {W, H, 3, 4, NULL, mat, palette}, T_normal = {W, H, 0, 3, normal, mat, palette};
+    const taa_tex T_depth = {W, H, 0, 1, depth, mat, palette};
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+    for (int64_t ii = 0; ii < (int64_t)W * H; ++ii) {
+        const uint32_t i = (uint32_t)ii;
+        const int32_t px = (int32_t)(i % (uint32_t)W), py = (int32_t)(i / (uint32_t)W);
+        const float u = ((float)px + 0.5f) / (float)W, v = ((float)py + 0.5f) / (float)H;
+        const float irx = 1.0f / (float)W, iry = 1.0f / (float)H;
+        float mo[2], cur[4], col[3], matl[4], nrm[3], dep, lst[4];
+        float* o = out + 4 * (size_t)i;
+        taa_sample(&T_motion, u, v, 2, mo);
+        const float ou = u + mo[0], ov = v + mo[1];
+        taa_sample(&T_light, u, v, 4, cur);
+        taa_sample(&T_color, u, v, 3, col);
+        taa_sample(&T_matl, u, v, 4, matl);
+        taa_sample(&T_normal, u, v, 3, nrm);
+        taa_sample(&T_depth, u, v, 1, &dep);
+        if (dep == 1.0f) {                                       /* LightTAA.frag:50-53 */
+            for (int c = 0; c < 4; ++c) o[c] = cur[c];
+            continue;
+        }
+        taa_sample(&T_last, ou, ov, 4, lst);
+        float lvar = lst[3];
+        if (g_clamp(ou, 0.0f, 1.0f) != ou || g_clamp(ov, 0.0f, 1.0f) != ov) {   /* LightTAA.frag:58-83 */
+            float count = 0.0f, acc[3] = {0.0f, 0.0f, 0.0f};
+            for (int x = -9; x <= 9; ++x)
+                for (int y = -9; y <= 9; ++y) {
+                    const float su = g_clamp(u + (float)x * irx, 0.001f, 0.999f);
+                    const float sv = g_clamp(v + (float)y * iry, 0.001f, 0.999f);
+                    float nm[2], nc[3], nn[3], nmat[4], nd, nl[3], dd[4];
+                    taa_sample(&T_motion, su, sv, 2, nm);
+                    taa_sample(&T_color, su, sv, 3, nc);
+                    taa_sample(&T_normal, su, sv, 3, nn);
+                    taa_sample(&T_matl, su, sv, 4, nmat);
+                    taa_sample(&T_depth, su, sv, 1, &nd);
+                    float f = g_max((nrm[0] * nn[0] + nrm[1] * nn[1]) + nrm[2] * nn[2], 0.0f);
+                    for (int k = 0; k < 4; ++k) dd[k] = matl[k] - nmat[k];
+                    f *= g_step(0.8f, 1.0f - taa_len(dd, 4));
+                    f *= 1.0f - g_clamp(fabsf(dep - nd) * 4096.0f, 0.0f, 1.0f);
+                    for (int k = 0; k < 3; ++k) dd[k] = col[k] - nc[k];
+                    f *= 1.0f - g_clamp(taa_len(dd, 3), 0.0f, 1.0f);
+                    dd[0] = mo[0] - nm[0]; dd[1] = mo[1] - nm[1];
+                    if (taa_len(dd, 2) > 0.1f) f = 0.0f;
+                    taa_sample(&T_light, su, sv, 3, nl);
+                    for (int k = 0; k < 3; ++k) acc[k] = acc[k] + f * nl[k];
+                    count = count + f;
+                }
+            o[0] = acc[0] / count; o[1] = acc[1] / count; o[2] = acc[2] / count; o[3] = 1.0f;
+            continue;
+        }
+        /* LightTAA.frag:87-137 */
+        float nmin[3] = {10000.0f, 10000.0f, 10000.0f}, nmax[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f};
+        float diff = 1.0f, count = 0.0f, radius = 1.0f;
+        const float size = 12.0f;
+        const float noise = vxo_rand(vxo_pixkey(seed ^ 0x85ebca6bu, frame, i), 0u);
+        const float scale = g_clamp(0.1f, 0.5f, 1.0f / dep);
+        for (float angle = (noise * 3.1415f) * 2.118033988749895f; radius <= size; angle = angle + 2.39f) {
+            radius = radius + 1.0f;
+            float sn, cs, nm[2], nc[3], nn[3], nmat[4], nd, nl[3], dd[4];
+            vxo_sincos(angle, &sn, &cs);
+            const float rr = radius * (lvar + 1.0f);
+            const float su = g_clamp(u + ((cs * irx) * rr) * scale, 0.001f, 0.999f);
+            const float sv = g_clamp(v + ((sn * iry) * rr) * scale, 0.001f, 0.999f);
+            taa_sample(&T_motion, su, sv, 2, nm);
+            taa_sample(&T_color, su, sv, 3, nc);
+            taa_sample(&T_normal, su, sv, 3, nn);
+            taa_sample(&T_matl, su, sv, 4, nmat);
+            taa_sample(&T_depth, su, sv, 1, &nd);
+            float f = 1.212f - radius / size;
+            for (int k = 0; k < 4; ++k) dd[k] = matl[k] - nmat[k];
+            f *= g_step(0.8f, 1.0f - taa_len(dd, 4));
+            f *= g_max((nrm[0] * nn[0] + nrm[1] * nn[1]) + nrm[2] * nn[2], 0.0f);
+            f *= 1.0f - g_clamp(fabsf(dep - nd) * 4096.0f, 0.0f, 1.0f);
+            for (int k = 0; k < 3; ++k) dd[k] = col[k] - nc[k];
+            f *= 1.0f - g_clamp(taa_len(dd, 3) * 10000.0f, 0.0f, 1.0f);
+            dd[0] = mo[0] - nm[0]; dd[1] = mo[1] - nm[1];
+            if (taa_len(dd, 2) > 0.1f) f = 0.0f;
+            taa_sample(&T_light, su, sv, 3, nl);
+            for (int k = 0; k < 3; ++k) {
+                nmin[k] = g_min(nmin[k], nl[k]);
+                nmax[k] = g_max(nmax[k], nl[k]);
+                acc[k] = acc[k] + f * nl[k];
+                dd[k] = nl[k] - cur[k];
+            }
+            count = count + f;
+            diff = diff + taa_len(dd, 3) * f;
+        }
+        float d[3];
+        for (int k = 0; k < 3; ++k) cur[k] = (cur[k] + acc[k]) / (count + 1.0f);
+        diff = diff / radius;
+        for (int k = 0; k < 3; ++k) {
+            lst[k] = g_clamp(lst[k], nmin[k], nmax[k]);
+            d[k] = fabsf(cur[k] - lst[k]);
+        }
+        float var = (d[0] * 0.2125f + d[1] * 0.7154f) + d[2] * 0.0721f;   /* luminance, frag:31-36 */
+        var = g_clamp(var * 5.5f, 0.0f, 1.0f);
+        lvar = lvar + var;
+        lvar = lvar - diff * 0.08f;
+        {
+            float m2[2] = {mo[0], mo[1]};
+            lvar = lvar + taa_len(m2, 2) * 20.0f;
+        }
+        const float t = g_clamp(1.0f - lvar, 0.3f, 0.9f);
+        for (int k = 0; k < 3; ++k) o[k] = cur[k] * (1.0f - t) + lst[k] * t;
+        o[3] = g_clamp(lvar * 0.7f, 0.0f, 1.0f);
+    }
+    (void)nthreads;
+}
+
 /* Tangent frame of calculateAmbientIrradiance, LightAmbient.frag:112-113. */
 static inline void tangent_frame(const float n[3], float t[3], float b[3]) {
     if (fabsf(n[2]) > 0.5f) { t[0] = 0.0f; t[1] = -n[2]; t[2] = n[1]; }

@@ -142,6 +142,15 @@ typedef struct {
  * pixel. motion / depth / palette may be NULL. */
 void vxo_primary_instances(const vxo_vox_cmd* cmds, int32_t n, const vxo_view* view, float* pos, float* nrm,
                            uint8_t* mat, float* motion, float* depth, uint16_t* palette, int32_t nthreads);
+/* LightTAA (LightTAA.frag:37-141) over width x height pixels. light: 3 per
+ * pixel, last / out: 4, motion: 2 or NULL, depth: 1, normal: 3, mat: index,
+ * palette: 256 x 6. */
+void vxo_light_taa(int32_t width, int32_t height, int32_t frame, uint32_t seed, const float* light,
+                   const float* last, const float* motion, const float* depth, const float* normal,
+                   const uint8_t* mat, const float* palette, float* out, int32_t nthreads);
+/* The TAA sampler: bilinear, 8-bit sub-texel weights, opaque-black border,
+ * over an n-channel float image (test hook). */
+void vxo_taa_sample(const float* img, int32_t width, int32_t height, int32_t n, float u, float v, float* out);
 /* pixels: optional list of pixel indices to process (NULL = all). */
 void vxo_trace(const vxo_occ* v, const vxo_view* view, const vxo_trace_params* p,
                const float* pos, const float* nrm, const uint8_t* mat, const float* palette,

@@ -103,6 +103,11 @@ class GBufExt(C.Structure):
     _fields_ = [("motion", P), ("depth", P), ("palette", P)]
 
 
+class TaaIO(C.Structure):
+    _fields_ = [("light", P), ("last", P), ("motion", P), ("depth", P), ("normal", P), ("material", P),
+                ("palette", P), ("out", P)]
+
+
 class PrefabEntity(C.Structure):
     _fields_ = [("id", C.c_int32), ("parent", C.c_int32), ("flags", C.c_int32), ("model", C.c_int32),
                 ("palette", C.c_int32), ("light_type", C.c_int32), ("vox_guid", C.c_uint64),
@@ -173,6 +178,7 @@ SIGNATURES = {
     "vx_prefab_lights": ([P, C.POINTER(Light), C.c_uint32, C.POINTER(C.c_uint32)], C.c_int32),
     "vx_prefab_vox_cmds": ([P, C.POINTER(P), C.c_uint32, C.POINTER(C.c_float), C.POINTER(VoxCmd), C.c_uint32,
                             C.POINTER(C.c_uint32)], C.c_int32),
+    "vx_light_taa": ([P, C.POINTER(View), C.POINTER(TaaIO), C.c_uint32, P], C.c_int32),
     "vx_primary_instances": ([P, C.POINTER(VoxCmd), C.c_uint32, C.POINTER(View), C.POINTER(GBuf),
                               C.POINTER(GBufExt), P], C.c_int32),
 }

@@ -265,6 +265,16 @@ def primary_instances(ctx: Context, cmds, vd: dict, gbuf: GBuffer, ext: GBufferE
                                        C.byref(e) if e is not None else None, _stream(stream)))
 
 
+def light_taa(ctx: Context, vd: dict, light, last, out, depth, normal, material, palette, motion=None, seed=0,
+              stream=None):
+    """LightTAA: out (npx x 4) from this frame's light (npx x 3), last frame's
+    output (npx x 4), the G-buffer's depth / normal / material and motion."""
+    io = L.TaaIO(_ptr(light), _ptr(last), _ptr(motion), _ptr(depth), _ptr(normal), _ptr(material), _ptr(palette),
+                 _ptr(out))
+    v = view_struct(vd)
+    check(L.lib().vx_light_taa(ctx.h, C.byref(v), C.byref(io), seed & 0xFFFFFFFF, _stream(stream)))
+
+
 class Visibility:
     def __init__(self, width, height, fetches=False, device="cuda"):
         n = width * height

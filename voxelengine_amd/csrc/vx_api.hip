@@ -1042,6 +1042,27 @@ extern "C" vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, u
     return VX_OK;
 }
 
+extern "C" vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa_io* io, uint32_t seed,
+                                  void* stream) {
+    VX_REQUIRE(ctx && io, "ctx/io is NULL");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(io->light && io->last && io->depth && io->normal && io->material && io->palette && io->out,
+               "io has a NULL pointer (only motion may be NULL)");
+    VX_REQUIRE((const void*)io->out != (const void*)io->last, "out must not alias last");
+    if (vx_status st = use_device(ctx)) return st;
+    TaaArgs A;
+    A.width = view->width;
+    A.height = view->height;
+    A.frame = view->frame;
+    A.seed = seed;
+    A.light = io->light; A.last = io->last; A.motion = io->motion; A.depth = io->depth;
+    A.normal = io->normal; A.mat = io->material; A.palette = io->palette; A.out = io->out;
+    const uint32_t npx = (uint32_t)view->width * (uint32_t)view->height;
+    hipLaunchKernelGGL(k_light_taa, dim3(cdiv(npx, 256)), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_light_taa");
+    return VX_OK;
+}
+
 static vx_status persistent_grid(vx_ctx* ctx, const void* kernel, int block, size_t lds, int want_per_cu,
                                  uint32_t& grid) {
     int cus = 0, per_cu = 0;

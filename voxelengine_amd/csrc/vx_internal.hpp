@@ -125,6 +125,21 @@ struct PrimaryInstArgs {
     uint16_t* palette;
 };
 
+// LightTAA (LightTAA.frag): per-pixel attributes of this frame and the last.
+struct TaaArgs {
+    int32_t width, height;
+    int32_t frame;
+    uint32_t seed;
+    const float* light;       // 3 per pixel
+    const float* last;        // 4 per pixel
+    const float* motion;      // 2 per pixel or NULL
+    const float* depth;       // 1 per pixel
+    const float* normal;      // 3 per pixel
+    const uint8_t* mat;       // material index per pixel
+    const float* palette;     // 256 x 6
+    float* out;               // 4 per pixel
+};
+
 struct PrimaryArgs {
     vx_view view;
     TileMap tiles;
@@ -173,6 +188,7 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
 __global__ void __launch_bounds__(256) k_bin_instances(BinArgs A);
 __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A);
+__global__ void __launch_bounds__(256) k_light_taa(TaaArgs A);
 __global__ void k_shade_fwd(ShadeArgs A);
 __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);
 __global__ void __launch_bounds__(256) k_shade_bwd_batch(const ShadeArgs* items, const uint32_t* block0, uint32_t n,

@@ -1489,6 +1489,173 @@ __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A) {
 }
 
 // ===========================================================================
+// Temporal accumulation of the light buffer: LightTAA.frag:37-141
+// (LightTAAPipeline.h:35-51, WorldRenderer.cpp:262-267).
+//
+// Textures are sampled as the reference's samplers do (linear filter, clamp
+// to an opaque-black border, evk.cpp:278-292): bilinear over the four nearest
+// texels with 8-bit sub-texel weights; border texels read 0 in rgb and 1 in
+// alpha. Colour and material come from the palette by the pixel's material
+// index (the G-buffer colour / material attachments).
+// ===========================================================================
+namespace taa {
+enum Attr { LIGHT, LAST, MOTION, COLOR, MATL, NORMAL, DEPTH };
+
+// Texel (x, y) of attribute a, channel c; outside the image: the border.
+VXD float texel(const TaaArgs& A, int a, int32_t x, int32_t y, int c) {
+    if (x < 0 || y < 0 || x >= A.width || y >= A.height) return c == 3 ? 1.0f : 0.0f;
+    const size_t i = (size_t)y * A.width + x;
+    switch (a) {
+        case LIGHT: return c < 3 ? A.light[3 * i + c] : (A.mat[i] == 0 ? 1.0f : 0.0f);
+        case LAST: return A.last[4 * i + c];
+        case MOTION: return A.motion ? A.motion[2 * i + c] : 0.0f;
+        case COLOR: return A.mat[i] ? A.palette[6 * A.mat[i] + c] : 0.0f;
+        case MATL: return (A.mat[i] && c < 3) ? A.palette[6 * A.mat[i] + 3 + c] : 0.0f;
+        case NORMAL: return A.normal[3 * i + c];
+        default: return A.depth[i];
+    }
+}
+
+// Sub-texel weight: the fraction rounded to 1/256; 1 moves to the next texel.
+VXD void weight(float x, int32_t& x0, float& f) {
+    float fl = floorf(x);
+    f = floorf((x - fl) * 256.0f + 0.5f) * (1.0f / 256.0f);
+    if (f >= 1.0f) {
+        fl = fl + 1.0f;
+        f = 0.0f;
+    }
+    x0 = (int32_t)fl;
+}
+
+// texture(T, uv) of n channels.
+VXD void sample(const TaaArgs& A, int a, float u, float v, int n, float* out) {
+    int32_t x0, y0;
+    float fx, fy;
+    weight(u * (float)A.width - 0.5f, x0, fx);
+    weight(v * (float)A.height - 0.5f, y0, fy);
+    for (int c = 0; c < n; ++c) {
+        const float t00 = texel(A, a, x0, y0, c), t10 = texel(A, a, x0 + 1, y0, c);
+        const float t01 = texel(A, a, x0, y0 + 1, c), t11 = texel(A, a, x0 + 1, y0 + 1, c);
+        out[c] = (t00 * (1.0f - fx) + t10 * fx) * (1.0f - fy) + (t01 * (1.0f - fx) + t11 * fx) * fy;
+    }
+}
+
+VXD float len2(float x, float y) { return sqrtf(x * x + y * y); }
+VXD float len3(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z); }
+VXD float len4(const float* a, const float* b) {
+    const float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2], d3 = a[3] - b[3];
+    return sqrtf(((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3);
+}
+}  // namespace taa
+
+__global__ void __launch_bounds__(256) k_light_taa(TaaArgs A) {
+    using namespace taa;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)(A.width * A.height)) return;
+    const int32_t px = (int32_t)(i % (uint32_t)A.width), py = (int32_t)(i / (uint32_t)A.width);
+    const float W = (float)A.width, H = (float)A.height;
+    const float u = ((float)px + 0.5f) / W, v = ((float)py + 0.5f) / H;
+    const float irx = 1.0f / W, iry = 1.0f / H;
+    float motion[2], cur[4], col[3], matl[4], nrm[3], dep;
+    sample(A, MOTION, u, v, 2, motion);
+    const float ou = u + motion[0], ov = v + motion[1];
+    sample(A, LIGHT, u, v, 4, cur);
+    sample(A, COLOR, u, v, 3, col);
+    sample(A, MATL, u, v, 4, matl);
+    sample(A, NORMAL, u, v, 3, nrm);
+    sample(A, DEPTH, u, v, 1, &dep);
+    float* o = A.out + 4 * (size_t)i;
+    if (dep == 1.0f) {
+        o[0] = cur[0]; o[1] = cur[1]; o[2] = cur[2]; o[3] = cur[3];
+        return;
+    }
+    float last[4];
+    sample(A, LAST, ou, ov, 4, last);
+    float lvar = last[3];
+    if (gclamp(ou, 0.0f, 1.0f) != ou || gclamp(ov, 0.0f, 1.0f) != ov) {
+        // out of bounds: this frame's neighbourhood only
+        float count = 0.0f, acc[3] = {0.0f, 0.0f, 0.0f};
+        for (int x = -9; x <= 9; ++x) {
+            for (int y = -9; y <= 9; ++y) {
+                const float su = gclamp(u + (float)x * irx, 0.001f, 0.999f);
+                const float sv = gclamp(v + (float)y * iry, 0.001f, 0.999f);
+                float nm[2], nc[3], nn[3], nmat[4], nd, nl[3];
+                sample(A, MOTION, su, sv, 2, nm);
+                sample(A, COLOR, su, sv, 3, nc);
+                sample(A, NORMAL, su, sv, 3, nn);
+                sample(A, MATL, su, sv, 4, nmat);
+                sample(A, DEPTH, su, sv, 1, &nd);
+                float f = gmax((nrm[0] * nn[0] + nrm[1] * nn[1]) + nrm[2] * nn[2], 0.0f);
+                f *= gstep(0.8f, 1.0f - len4(matl, nmat));
+                f *= 1.0f - gclamp(fabsf(dep - nd) * 4096.0f, 0.0f, 1.0f);
+                f *= 1.0f - gclamp(len3(col[0] - nc[0], col[1] - nc[1], col[2] - nc[2]), 0.0f, 1.0f);
+                if (len2(motion[0] - nm[0], motion[1] - nm[1]) > 0.1f) f = 0.0f;
+                sample(A, LIGHT, su, sv, 3, nl);
+                acc[0] = acc[0] + f * nl[0];
+                acc[1] = acc[1] + f * nl[1];
+                acc[2] = acc[2] + f * nl[2];
+                count = count + f;
+            }
+        }
+        o[0] = acc[0] / count; o[1] = acc[1] / count; o[2] = acc[2] / count; o[3] = 1.0f;
+        return;
+    }
+    float nmin[3] = {10000.0f, 10000.0f, 10000.0f}, nmax[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f};
+    float diff = 1.0f, count = 0.0f;
+    const float size = 12.0f;
+    float radius = 1.0f;
+    const float noise = rand01(pixkey(A.seed ^ 0x85ebca6bu, A.frame, i), 0u);
+    const float scale = gclamp(0.1f, 0.5f, 1.0f / dep);
+    for (float angle = (noise * 3.1415f) * 2.118033988749895f; radius <= size; angle = angle + 2.39f) {
+        radius = radius + 1.0f;
+        float sn, cs;
+        sincos_vx(angle, &sn, &cs);
+        const float rr = radius * (lvar + 1.0f);
+        const float su = gclamp(u + ((cs * irx) * rr) * scale, 0.001f, 0.999f);
+        const float sv = gclamp(v + ((sn * iry) * rr) * scale, 0.001f, 0.999f);
+        float nm[2], nc[3], nn[3], nmat[4], nd, nl[3];
+        sample(A, MOTION, su, sv, 2, nm);
+        sample(A, COLOR, su, sv, 3, nc);
+        sample(A, NORMAL, su, sv, 3, nn);
+        sample(A, MATL, su, sv, 4, nmat);
+        sample(A, DEPTH, su, sv, 1, &nd);
+        float f = 1.212f - radius / size;
+        f *= gstep(0.8f, 1.0f - len4(matl, nmat));
+        f *= gmax((nrm[0] * nn[0] + nrm[1] * nn[1]) + nrm[2] * nn[2], 0.0f);
+        f *= 1.0f - gclamp(fabsf(dep - nd) * 4096.0f, 0.0f, 1.0f);
+        f *= 1.0f - gclamp(len3(col[0] - nc[0], col[1] - nc[1], col[2] - nc[2]) * 10000.0f, 0.0f, 1.0f);
+        if (len2(motion[0] - nm[0], motion[1] - nm[1]) > 0.1f) f = 0.0f;
+        sample(A, LIGHT, su, sv, 3, nl);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            nmin[c] = gmin(nmin[c], nl[c]);
+            nmax[c] = gmax(nmax[c], nl[c]);
+            acc[c] = acc[c] + f * nl[c];
+        }
+        count = count + f;
+        diff = diff + len3(nl[0] - cur[0], nl[1] - cur[1], nl[2] - cur[2]) * f;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cur[c] = (cur[c] + acc[c]) / (count + 1.0f);
+    diff = diff / radius;
+    float d[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        last[c] = gclamp(last[c], nmin[c], nmax[c]);
+        d[c] = fabsf(cur[c] - last[c]);
+    }
+    float var = (d[0] * 0.2125f + d[1] * 0.7154f) + d[2] * 0.0721f;
+    var = gclamp(var * 5.5f, 0.0f, 1.0f);
+    lvar = lvar + var;
+    lvar = lvar - diff * 0.08f;
+    lvar = lvar + len2(motion[0], motion[1]) * 20.0f;
+    const float t = gclamp(1.0f - lvar, 0.3f, 0.9f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = cur[c] * (1.0f - t) + last[c] * t;
+    o[3] = gclamp(lvar * 0.7f, 0.0f, 1.0f);
+}
+
+// ===========================================================================
 // Shading (LightAmbient.frag:178-214, PBRDirectLight PBR.frag:48-68,
 // fresnelSchlickRoughness PBR.frag:8-10) and its adjoint.
 // ===========================================================================
