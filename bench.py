@@ -217,9 +217,25 @@ def _prefab_frame(ctx, P, models, vol, reps=10):
     e1.record()
     torch.cuda.synchronize()
     tr_ms = e0.elapsed_time(e1) / reps
+    # shading of the frame, then the temporal accumulation over it (history fed back)
+    rgb = torch.zeros((W * H, 3), dtype=torch.float32, device="cuda")
+    api.shade_forward(ctx, vd, g, vis, pal_d, configs.default_light(), rgb)
+    hist = [torch.zeros((W * H, 4), dtype=torch.float32, device="cuda") for _ in range(2)]
+    for k in range(3):
+        api.light_taa(ctx, vd, rgb, hist[k % 2], hist[(k + 1) % 2], ext.depth, g.normal, g.material, pal_d,
+                      motion=ext.motion, seed=1)
+    torch.cuda.synchronize()
+    e0.record()
+    for k in range(reps):
+        api.light_taa(ctx, vd, rgb, hist[k % 2], hist[(k + 1) % 2], ext.depth, g.normal, g.material, pal_d,
+                      motion=ext.motion, seed=1)
+    e1.record()
+    torch.cuda.synchronize()
+    taa_ms = e0.elapsed_time(e1) / reps
     return {"draws": len(cmds), "hit_pixels": hit, "geometry_ms": round(geo_ms, 3),
             "geometry_mpx_per_s": round(W * H / (geo_ms * 1e-3) / 1e6, 1), "trace_ms": round(tr_ms, 3),
-            "trace_grays_per_s": round(rays / (tr_ms * 1e-3) / 1e9, 3)}
+            "trace_grays_per_s": round(rays / (tr_ms * 1e-3) / 1e9, 3), "taa_ms": round(taa_ms, 3),
+            "taa_mpx_per_s": round(W * H / (taa_ms * 1e-3) / 1e6, 1)}
 
 
 def bench_prefabs(ctx, reps=5):

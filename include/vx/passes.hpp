@@ -408,4 +408,18 @@ private:
     std::vector<vx_light> lights_;
 };
 
+// LightTAAPipeline (LightTAAPipeline.h:10-57): Use(cmd, viewBuffer, blueNoise,
+// LastLightTAAFB, LightFB, GeometryFB) becomes one call over the device
+// buffers; the caller swaps out and last between frames (WorldRenderer.cpp:140).
+class LightTAAPipeline {
+public:
+    static LightTAAPipeline& Get() {
+        static LightTAAPipeline p;
+        return p;
+    }
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_taa_io& io, uint32_t seed = 0) {
+        check(vx_light_taa(ctx.get(), &viewBuffer, &io, seed, cmd));
+    }
+};
+
 }  // namespace vx
