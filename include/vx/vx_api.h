@@ -341,7 +341,7 @@ vx_status vx_light_pass(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, 
 /* LightTAA inputs and output (device; width*height pixels, rows from the top). */
 typedef struct {
     const float* light;        /* 3 per pixel: this frame's light buffer (ambient + lights) */
-    const float* last;         /* 4 per pixel: last frame's TAA output (rgb, variance) */
+    const float* last;         /* 4 per pixel: last frame's TAA output (rgb, variance); 16-byte aligned */
     const float* motion;       /* 2 per pixel (vx_gbuf_ext.motion); NULL = no motion */
     const float* depth;        /* 1 per pixel (vx_gbuf_ext.depth, 1 = sky) */
     const float* normal;       /* 3 per pixel (vx_gbuf.normal) */

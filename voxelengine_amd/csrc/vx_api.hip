@@ -101,6 +101,8 @@ struct vx_ctx {
     uint16_t* tile_lists;    // per-tile instance lists
     uint32_t* tile_counts;
     uint32_t tile_cap;
+    void* taa_rec;           // LightTAA: packed per-pixel attributes (64 B each)
+    uint32_t taa_cap;
 };
 
 static void profile_clear(vx_ctx* c) {
@@ -244,6 +246,8 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     c->tile_lists = nullptr;
     c->tile_counts = nullptr;
     c->tile_cap = 0;
+    c->taa_rec = nullptr;
+    c->taa_cap = 0;
     hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
     if (me == hipSuccess) me = hipMalloc(&c->march.counters, kCounterWords * sizeof(uint32_t));
     if (me != hipSuccess) {
@@ -267,6 +271,7 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     (void)hipFree(ctx->inst_rects);
     (void)hipFree(ctx->tile_lists);
     (void)hipFree(ctx->tile_counts);
+    (void)hipFree(ctx->taa_rec);
     profile_clear(ctx);
     delete ctx;
     return VX_OK;
@@ -1049,6 +1054,7 @@ extern "C" vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa
     VX_REQUIRE(io->light && io->last && io->depth && io->normal && io->material && io->palette && io->out,
                "io has a NULL pointer (only motion may be NULL)");
     VX_REQUIRE((const void*)io->out != (const void*)io->last, "out must not alias last");
+    VX_REQUIRE(((uintptr_t)io->last & 15u) == 0, "last must be 16-byte aligned");
     if (vx_status st = use_device(ctx)) return st;
     TaaArgs A;
     A.width = view->width;
@@ -1058,6 +1064,17 @@ extern "C" vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa
     A.light = io->light; A.last = io->last; A.motion = io->motion; A.depth = io->depth;
     A.normal = io->normal; A.mat = io->material; A.palette = io->palette; A.out = io->out;
     const uint32_t npx = (uint32_t)view->width * (uint32_t)view->height;
+    if (ctx->taa_cap < npx) {
+        VX_HIP(hipDeviceSynchronize());
+        (void)hipFree(ctx->taa_rec);
+        ctx->taa_rec = nullptr;
+        ctx->taa_cap = 0;
+        VX_HIP(hipMalloc(&ctx->taa_rec, (size_t)npx * 64));
+        ctx->taa_cap = npx;
+    }
+    A.rec = reinterpret_cast<float4*>(ctx->taa_rec);
+    hipLaunchKernelGGL(k_taa_pack, dim3(cdiv(npx, 256)), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_taa_pack");
     hipLaunchKernelGGL(k_light_taa, dim3(cdiv(npx, 256)), dim3(256), 0, S(stream), A);
     VX_LAUNCHED("k_light_taa");
     return VX_OK;

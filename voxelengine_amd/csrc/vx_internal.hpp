@@ -138,6 +138,7 @@ struct TaaArgs {
     const uint8_t* mat;       // material index per pixel
     const float* palette;     // 256 x 6
     float* out;               // 4 per pixel
+    float4* rec;              // 4 float4 per pixel: packed attributes (k_taa_pack)
 };
 
 struct PrimaryArgs {
@@ -188,6 +189,7 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
 __global__ void __launch_bounds__(256) k_bin_instances(BinArgs A);
 __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A);
+__global__ void __launch_bounds__(256) k_taa_pack(TaaArgs A);
 __global__ void __launch_bounds__(256) k_light_taa(TaaArgs A);
 __global__ void k_shade_fwd(ShadeArgs A);
 __global__ void __launch_bounds__(256) k_shade_bwd(ShadeArgs A, long long* acc);

@@ -1499,21 +1499,30 @@ __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A) {
 // index (the G-buffer colour / material attachments).
 // ===========================================================================
 namespace taa {
-enum Attr { LIGHT, LAST, MOTION, COLOR, MATL, NORMAL, DEPTH };
+// One pixel's attributes in one 64-byte record, so a bilinear sample of every
+// attribute loads four records (k_taa_pack writes them):
+//   v0 = light rgb, light alpha (1 on sky pixels)   v1 = colour rgb, depth
+//   v2 = material rgb, motion x                      v3 = normal xyz, motion y
+// The material's fourth channel is 0 inside the image and 1 on the border.
+struct Texel {
+    float4 v0, v1, v2, v3;
+};
 
-// Texel (x, y) of attribute a, channel c; outside the image: the border.
-VXD float texel(const TaaArgs& A, int a, int32_t x, int32_t y, int c) {
-    if (x < 0 || y < 0 || x >= A.width || y >= A.height) return c == 3 ? 1.0f : 0.0f;
-    const size_t i = (size_t)y * A.width + x;
-    switch (a) {
-        case LIGHT: return c < 3 ? A.light[3 * i + c] : (A.mat[i] == 0 ? 1.0f : 0.0f);
-        case LAST: return A.last[4 * i + c];
-        case MOTION: return A.motion ? A.motion[2 * i + c] : 0.0f;
-        case COLOR: return A.mat[i] ? A.palette[6 * A.mat[i] + c] : 0.0f;
-        case MATL: return (A.mat[i] && c < 3) ? A.palette[6 * A.mat[i] + 3 + c] : 0.0f;
-        case NORMAL: return A.normal[3 * i + c];
-        default: return A.depth[i];
-    }
+VXD Texel border_texel() {
+    Texel t;
+    t.v0 = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    t.v1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    t.v2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    t.v3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return t;
+}
+
+VXD Texel fetch(const TaaArgs& A, int32_t x, int32_t y) {
+    if (x < 0 || y < 0 || x >= A.width || y >= A.height) return border_texel();
+    const float4* r = A.rec + 4 * ((size_t)y * A.width + x);
+    Texel t;
+    t.v0 = r[0]; t.v1 = r[1]; t.v2 = r[2]; t.v3 = r[3];
+    return t;
 }
 
 // Sub-texel weight: the fraction rounded to 1/256; 1 moves to the next texel.
@@ -1527,17 +1536,58 @@ VXD void weight(float x, int32_t& x0, float& f) {
     x0 = (int32_t)fl;
 }
 
-// texture(T, uv) of n channels.
-VXD void sample(const TaaArgs& A, int a, float u, float v, int n, float* out) {
+VXD float bl(float a, float b, float c, float d, float fx, float fy) {
+    return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + d * fx) * fy;
+}
+
+// Every attribute of texture(*, uv).
+struct Sample {
+    float light[4], color[3], depth, matl[4], nrm[3], mo[2];
+};
+
+VXD void sample_all(const TaaArgs& A, float u, float v, Sample& s) {
     int32_t x0, y0;
     float fx, fy;
     weight(u * (float)A.width - 0.5f, x0, fx);
     weight(v * (float)A.height - 0.5f, y0, fy);
-    for (int c = 0; c < n; ++c) {
-        const float t00 = texel(A, a, x0, y0, c), t10 = texel(A, a, x0 + 1, y0, c);
-        const float t01 = texel(A, a, x0, y0 + 1, c), t11 = texel(A, a, x0 + 1, y0 + 1, c);
-        out[c] = (t00 * (1.0f - fx) + t10 * fx) * (1.0f - fy) + (t01 * (1.0f - fx) + t11 * fx) * fy;
+    const Texel a = fetch(A, x0, y0), b = fetch(A, x0 + 1, y0), c = fetch(A, x0, y0 + 1), d = fetch(A, x0 + 1, y0 + 1);
+    s.light[0] = bl(a.v0.x, b.v0.x, c.v0.x, d.v0.x, fx, fy);
+    s.light[1] = bl(a.v0.y, b.v0.y, c.v0.y, d.v0.y, fx, fy);
+    s.light[2] = bl(a.v0.z, b.v0.z, c.v0.z, d.v0.z, fx, fy);
+    s.light[3] = bl(a.v0.w, b.v0.w, c.v0.w, d.v0.w, fx, fy);
+    s.color[0] = bl(a.v1.x, b.v1.x, c.v1.x, d.v1.x, fx, fy);
+    s.color[1] = bl(a.v1.y, b.v1.y, c.v1.y, d.v1.y, fx, fy);
+    s.color[2] = bl(a.v1.z, b.v1.z, c.v1.z, d.v1.z, fx, fy);
+    s.depth = bl(a.v1.w, b.v1.w, c.v1.w, d.v1.w, fx, fy);
+    s.matl[0] = bl(a.v2.x, b.v2.x, c.v2.x, d.v2.x, fx, fy);
+    s.matl[1] = bl(a.v2.y, b.v2.y, c.v2.y, d.v2.y, fx, fy);
+    s.matl[2] = bl(a.v2.z, b.v2.z, c.v2.z, d.v2.z, fx, fy);
+    const auto out = [&](int32_t x, int32_t y) { return (x < 0 || y < 0 || x >= A.width || y >= A.height) ? 1.0f : 0.0f; };
+    s.matl[3] = bl(out(x0, y0), out(x0 + 1, y0), out(x0, y0 + 1), out(x0 + 1, y0 + 1), fx, fy);
+    s.mo[0] = bl(a.v2.w, b.v2.w, c.v2.w, d.v2.w, fx, fy);
+    s.nrm[0] = bl(a.v3.x, b.v3.x, c.v3.x, d.v3.x, fx, fy);
+    s.nrm[1] = bl(a.v3.y, b.v3.y, c.v3.y, d.v3.y, fx, fy);
+    s.nrm[2] = bl(a.v3.z, b.v3.z, c.v3.z, d.v3.z, fx, fy);
+    s.mo[1] = bl(a.v3.w, b.v3.w, c.v3.w, d.v3.w, fx, fy);
+}
+
+// texture(LAST_LIGHT_TEXTURE, uv): 4 channels, border (0, 0, 0, 1).
+VXD void sample_last(const TaaArgs& A, float u, float v, float* o) {
+    int32_t x0, y0;
+    float fx, fy;
+    weight(u * (float)A.width - 0.5f, x0, fx);
+    weight(v * (float)A.height - 0.5f, y0, fy);
+    float t[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t x = x0 + (k & 1), y = y0 + (k >> 1);
+        const bool in = !(x < 0 || y < 0 || x >= A.width || y >= A.height);
+        const float4 q = in ? reinterpret_cast<const float4*>(A.last)[(size_t)y * A.width + x]
+                            : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        t[k][0] = q.x; t[k][1] = q.y; t[k][2] = q.z; t[k][3] = q.w;
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[c] = bl(t[0][c], t[1][c], t[2][c], t[3][c], fx, fy);
 }
 
 VXD float len2(float x, float y) { return sqrtf(x * x + y * y); }
@@ -1548,6 +1598,21 @@ VXD float len4(const float* a, const float* b) {
 }
 }  // namespace taa
 
+// One record per pixel from the G-buffer, the palette and this frame's light.
+__global__ void __launch_bounds__(256) k_taa_pack(TaaArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)(A.width * A.height)) return;
+    const uint32_t m = A.mat[i];
+    const float* pe = A.palette + 6 * m;
+    const float mx = A.motion ? A.motion[2 * (size_t)i] : 0.0f, my = A.motion ? A.motion[2 * (size_t)i + 1] : 0.0f;
+    float4* r = A.rec + 4 * (size_t)i;
+    r[0] = make_float4(A.light[3 * (size_t)i], A.light[3 * (size_t)i + 1], A.light[3 * (size_t)i + 2],
+                       m == 0 ? 1.0f : 0.0f);
+    r[1] = make_float4(m ? pe[0] : 0.0f, m ? pe[1] : 0.0f, m ? pe[2] : 0.0f, A.depth[i]);
+    r[2] = make_float4(m ? pe[3] : 0.0f, m ? pe[4] : 0.0f, m ? pe[5] : 0.0f, mx);
+    r[3] = make_float4(A.normal[3 * (size_t)i], A.normal[3 * (size_t)i + 1], A.normal[3 * (size_t)i + 2], my);
+}
+
 __global__ void __launch_bounds__(256) k_light_taa(TaaArgs A) {
     using namespace taa;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1556,44 +1621,37 @@ __global__ void __launch_bounds__(256) k_light_taa(TaaArgs A) {
     const float W = (float)A.width, H = (float)A.height;
     const float u = ((float)px + 0.5f) / W, v = ((float)py + 0.5f) / H;
     const float irx = 1.0f / W, iry = 1.0f / H;
-    float motion[2], cur[4], col[3], matl[4], nrm[3], dep;
-    sample(A, MOTION, u, v, 2, motion);
-    const float ou = u + motion[0], ov = v + motion[1];
-    sample(A, LIGHT, u, v, 4, cur);
-    sample(A, COLOR, u, v, 3, col);
-    sample(A, MATL, u, v, 4, matl);
-    sample(A, NORMAL, u, v, 3, nrm);
-    sample(A, DEPTH, u, v, 1, &dep);
+    Sample c;
+    sample_all(A, u, v, c);
+    const float ou = u + c.mo[0], ov = v + c.mo[1];
     float* o = A.out + 4 * (size_t)i;
-    if (dep == 1.0f) {
-        o[0] = cur[0]; o[1] = cur[1]; o[2] = cur[2]; o[3] = cur[3];
+    if (c.depth == 1.0f) {
+        o[0] = c.light[0]; o[1] = c.light[1]; o[2] = c.light[2]; o[3] = c.light[3];
         return;
     }
     float last[4];
-    sample(A, LAST, ou, ov, 4, last);
+    sample_last(A, ou, ov, last);
     float lvar = last[3];
     if (gclamp(ou, 0.0f, 1.0f) != ou || gclamp(ov, 0.0f, 1.0f) != ov) {
         // out of bounds: this frame's neighbourhood only
         float count = 0.0f, acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
         for (int x = -9; x <= 9; ++x) {
+#pragma unroll 1
             for (int y = -9; y <= 9; ++y) {
                 const float su = gclamp(u + (float)x * irx, 0.001f, 0.999f);
                 const float sv = gclamp(v + (float)y * iry, 0.001f, 0.999f);
-                float nm[2], nc[3], nn[3], nmat[4], nd, nl[3];
-                sample(A, MOTION, su, sv, 2, nm);
-                sample(A, COLOR, su, sv, 3, nc);
-                sample(A, NORMAL, su, sv, 3, nn);
-                sample(A, MATL, su, sv, 4, nmat);
-                sample(A, DEPTH, su, sv, 1, &nd);
-                float f = gmax((nrm[0] * nn[0] + nrm[1] * nn[1]) + nrm[2] * nn[2], 0.0f);
-                f *= gstep(0.8f, 1.0f - len4(matl, nmat));
-                f *= 1.0f - gclamp(fabsf(dep - nd) * 4096.0f, 0.0f, 1.0f);
-                f *= 1.0f - gclamp(len3(col[0] - nc[0], col[1] - nc[1], col[2] - nc[2]), 0.0f, 1.0f);
-                if (len2(motion[0] - nm[0], motion[1] - nm[1]) > 0.1f) f = 0.0f;
-                sample(A, LIGHT, su, sv, 3, nl);
-                acc[0] = acc[0] + f * nl[0];
-                acc[1] = acc[1] + f * nl[1];
-                acc[2] = acc[2] + f * nl[2];
+                Sample n;
+                sample_all(A, su, sv, n);
+                float f = gmax((c.nrm[0] * n.nrm[0] + c.nrm[1] * n.nrm[1]) + c.nrm[2] * n.nrm[2], 0.0f);
+                f *= gstep(0.8f, 1.0f - len4(c.matl, n.matl));
+                f *= 1.0f - gclamp(fabsf(c.depth - n.depth) * 4096.0f, 0.0f, 1.0f);
+                f *= 1.0f - gclamp(len3(c.color[0] - n.color[0], c.color[1] - n.color[1], c.color[2] - n.color[2]),
+                                   0.0f, 1.0f);
+                if (len2(c.mo[0] - n.mo[0], c.mo[1] - n.mo[1]) > 0.1f) f = 0.0f;
+                acc[0] = acc[0] + f * n.light[0];
+                acc[1] = acc[1] + f * n.light[1];
+                acc[2] = acc[2] + f * n.light[2];
                 count = count + f;
             }
         }
@@ -1605,7 +1663,8 @@ __global__ void __launch_bounds__(256) k_light_taa(TaaArgs A) {
     const float size = 12.0f;
     float radius = 1.0f;
     const float noise = rand01(pixkey(A.seed ^ 0x85ebca6bu, A.frame, i), 0u);
-    const float scale = gclamp(0.1f, 0.5f, 1.0f / dep);
+    const float scale = gclamp(0.1f, 0.5f, 1.0f / c.depth);
+#pragma unroll 1
     for (float angle = (noise * 3.1415f) * 2.118033988749895f; radius <= size; angle = angle + 2.39f) {
         radius = radius + 1.0f;
         float sn, cs;
@@ -1613,45 +1672,41 @@ __global__ void __launch_bounds__(256) k_light_taa(TaaArgs A) {
         const float rr = radius * (lvar + 1.0f);
         const float su = gclamp(u + ((cs * irx) * rr) * scale, 0.001f, 0.999f);
         const float sv = gclamp(v + ((sn * iry) * rr) * scale, 0.001f, 0.999f);
-        float nm[2], nc[3], nn[3], nmat[4], nd, nl[3];
-        sample(A, MOTION, su, sv, 2, nm);
-        sample(A, COLOR, su, sv, 3, nc);
-        sample(A, NORMAL, su, sv, 3, nn);
-        sample(A, MATL, su, sv, 4, nmat);
-        sample(A, DEPTH, su, sv, 1, &nd);
+        Sample n;
+        sample_all(A, su, sv, n);
         float f = 1.212f - radius / size;
-        f *= gstep(0.8f, 1.0f - len4(matl, nmat));
-        f *= gmax((nrm[0] * nn[0] + nrm[1] * nn[1]) + nrm[2] * nn[2], 0.0f);
-        f *= 1.0f - gclamp(fabsf(dep - nd) * 4096.0f, 0.0f, 1.0f);
-        f *= 1.0f - gclamp(len3(col[0] - nc[0], col[1] - nc[1], col[2] - nc[2]) * 10000.0f, 0.0f, 1.0f);
-        if (len2(motion[0] - nm[0], motion[1] - nm[1]) > 0.1f) f = 0.0f;
-        sample(A, LIGHT, su, sv, 3, nl);
+        f *= gstep(0.8f, 1.0f - len4(c.matl, n.matl));
+        f *= gmax((c.nrm[0] * n.nrm[0] + c.nrm[1] * n.nrm[1]) + c.nrm[2] * n.nrm[2], 0.0f);
+        f *= 1.0f - gclamp(fabsf(c.depth - n.depth) * 4096.0f, 0.0f, 1.0f);
+        f *= 1.0f - gclamp(len3(c.color[0] - n.color[0], c.color[1] - n.color[1], c.color[2] - n.color[2]) * 10000.0f,
+                           0.0f, 1.0f);
+        if (len2(c.mo[0] - n.mo[0], c.mo[1] - n.mo[1]) > 0.1f) f = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            nmin[c] = gmin(nmin[c], nl[c]);
-            nmax[c] = gmax(nmax[c], nl[c]);
-            acc[c] = acc[c] + f * nl[c];
+        for (int k = 0; k < 3; ++k) {
+            nmin[k] = gmin(nmin[k], n.light[k]);
+            nmax[k] = gmax(nmax[k], n.light[k]);
+            acc[k] = acc[k] + f * n.light[k];
         }
         count = count + f;
-        diff = diff + len3(nl[0] - cur[0], nl[1] - cur[1], nl[2] - cur[2]) * f;
+        diff = diff + len3(n.light[0] - c.light[0], n.light[1] - c.light[1], n.light[2] - c.light[2]) * f;
     }
+    float cur[3], d[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) cur[c] = (cur[c] + acc[c]) / (count + 1.0f);
+    for (int k = 0; k < 3; ++k) cur[k] = (c.light[k] + acc[k]) / (count + 1.0f);
     diff = diff / radius;
-    float d[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        last[c] = gclamp(last[c], nmin[c], nmax[c]);
-        d[c] = fabsf(cur[c] - last[c]);
+    for (int k = 0; k < 3; ++k) {
+        last[k] = gclamp(last[k], nmin[k], nmax[k]);
+        d[k] = fabsf(cur[k] - last[k]);
     }
     float var = (d[0] * 0.2125f + d[1] * 0.7154f) + d[2] * 0.0721f;
     var = gclamp(var * 5.5f, 0.0f, 1.0f);
     lvar = lvar + var;
     lvar = lvar - diff * 0.08f;
-    lvar = lvar + len2(motion[0], motion[1]) * 20.0f;
+    lvar = lvar + len2(c.mo[0], c.mo[1]) * 20.0f;
     const float t = gclamp(1.0f - lvar, 0.3f, 0.9f);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o[c] = cur[c] * (1.0f - t) + last[c] * t;
+    for (int k = 0; k < 3; ++k) o[k] = cur[k] * (1.0f - t) + last[k] * t;
     o[3] = gclamp(lvar * 0.7f, 0.0f, 1.0f);
 }
 
