@@ -710,8 +710,12 @@ static float taa_texel(const taa_tex* t, int32_t x, int32_t y, int c) {
     }
 }
 
-/* Position x in texels to the left texel and the 8-bit weight of its right neighbour. */
-static void taa_weight(float x, int32_t* x0, float* f) {
+/* Position x in texels to the left texel and the 8-bit weight of its right
+ * neighbour. Positions left of -2 or right of size + 1 (and NaN) read border
+ * texels only and are clamped first. */
+static void taa_weight(float x, int32_t size, int32_t* x0, float* f) {
+    if (!(x >= -2.0f)) x = -2.0f;
+    if (x > (float)size + 1.0f) x = (float)size + 1.0f;
     float fl = floorf(x);
     float w = floorf((x - fl) * 256.0f + 0.5f) * (1.0f / 256.0f);
     if (w >= 1.0f) { fl = fl + 1.0f; w = 0.0f; }
@@ -722,8 +726,8 @@ static void taa_weight(float x, int32_t* x0, float* f) {
 static void taa_sample(const taa_tex* t, float u, float v, int n, float* out) {
     int32_t x0, y0;
     float fx, fy;
-    taa_weight(u * (float)t->w - 0.5f, &x0, &fx);
-    taa_weight(v * (float)t->h - 0.5f, &y0, &fy);
+    taa_weight(u * (float)t->w - 0.5f, t->w, &x0, &fx);
+    taa_weight(v * (float)t->h - 0.5f, t->h, &y0, &fy);
     for (int c = 0; c < n; ++c) {
         float a = taa_texel(t, x0, y0, c), b = taa_texel(t, x0 + 1, y0, c);
         float cc = taa_texel(t, x0, y0 + 1, c), d = taa_texel(t, x0 + 1, y0 + 1, c);

@@ -50,3 +50,33 @@ def test_taa_frames_match_oracle(ctx, oracle, tmp_path):
         ood = max(ood, int((hit & ((u < 0) | (u > 1) | (v < 0) | (v > 1))).sum()))
         last = out
     assert ood > 50     # the jump sends pixels through the off-screen branch
+
+
+def test_taa_extreme_motion_matches_oracle(ctx, oracle):
+    """History positions far off the image, infinite or NaN (the motion of a
+    draw whose last position projects behind the camera) read the border,
+    the same on the device as in the oracle."""
+    import torch
+    from voxelengine_amd import api, configs
+    W, H = 96, 64
+    npx = W * H
+    rng = np.random.default_rng(3)
+    n = rng.normal(size=(npx, 3)).astype(np.float32)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    mat = rng.integers(1, 4, npx).astype(np.uint8)
+    mat[rng.random(npx) < 0.1] = 0
+    depth = np.where(mat > 0, rng.uniform(0.001, 0.02, npx), 1.0).astype(np.float32)
+    motion = (rng.normal(size=(npx, 2)) * 0.01).astype(np.float32)
+    special = np.array([np.inf, -np.inf, np.nan, 1e30, -1e30, 3e9, -3e9, 0.6, -0.6], np.float32)
+    pick = rng.random(npx) < 0.3
+    motion[pick, rng.integers(0, 2, pick.sum())] = special[rng.integers(0, len(special), pick.sum())]
+    light = rng.random((npx, 3)).astype(np.float32)
+    last = rng.random((npx, 4)).astype(np.float32)
+    pal = configs.make_palette()
+    vd = views.make_view((0.0, 2.0, 0.0), (1.0, 0.0, 1.0), W, H, frame=2)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    out = torch.zeros((npx, 4), dtype=torch.float32, device="cuda")
+    api.light_taa(ctx, vd, d(light), d(last), out, d(depth), d(n), d(mat), d(pal), motion=d(motion), seed=4)
+    torch.cuda.synchronize()
+    ref = oracle.light_taa(W, H, 2, 4, light, last, motion, depth, n, mat, pal)
+    assert_bitwise(out.cpu().numpy(), ref, "taa")

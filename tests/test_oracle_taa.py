@@ -109,3 +109,10 @@ def test_other_surfaces_do_not_leak_across_a_material_edge(oracle):
     np.testing.assert_allclose(out[rows, N // 2:N // 2 + 3, :3], np.broadcast_to(LB, (20, 3, 3)), rtol=2e-6)
     # the running variance there still follows the same-surface closed form
     np.testing.assert_allclose(out[rows, N // 2 - 3:N // 2, 3], (0.2 - 0.08 / 13.0) * 0.7, rtol=1e-5)
+
+
+def test_sampler_far_and_non_finite_positions_read_the_border(oracle):
+    img = np.ones((5, 7, 4), np.float32)
+    for u, v in [(1e30, 0.5), (-1e30, 0.5), (np.inf, 0.5), (-np.inf, 0.5), (np.nan, 0.5), (0.5, np.nan),
+                 (0.5, 3e9), (2.0, 0.5)]:
+        np.testing.assert_array_equal(oracle.taa_sample(img, np.float32(u), np.float32(v)), [0, 0, 0, 1])

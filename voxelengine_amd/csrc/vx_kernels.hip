@@ -1526,7 +1526,11 @@ VXD Texel fetch(const TaaArgs& A, int32_t x, int32_t y) {
 }
 
 // Sub-texel weight: the fraction rounded to 1/256; 1 moves to the next texel.
-VXD void weight(float x, int32_t& x0, float& f) {
+// Positions left of -2 or right of size + 1 (and NaN) sample border texels
+// only; they are clamped first so that x0 and x0 + 1 stay small integers.
+VXD void weight(float x, int32_t size, int32_t& x0, float& f) {
+    if (!(x >= -2.0f)) x = -2.0f;
+    if (x > (float)size + 1.0f) x = (float)size + 1.0f;
     float fl = floorf(x);
     f = floorf((x - fl) * 256.0f + 0.5f) * (1.0f / 256.0f);
     if (f >= 1.0f) {
@@ -1548,8 +1552,8 @@ struct Sample {
 VXD void sample_all(const TaaArgs& A, float u, float v, Sample& s) {
     int32_t x0, y0;
     float fx, fy;
-    weight(u * (float)A.width - 0.5f, x0, fx);
-    weight(v * (float)A.height - 0.5f, y0, fy);
+    weight(u * (float)A.width - 0.5f, A.width, x0, fx);
+    weight(v * (float)A.height - 0.5f, A.height, y0, fy);
     const Texel a = fetch(A, x0, y0), b = fetch(A, x0 + 1, y0), c = fetch(A, x0, y0 + 1), d = fetch(A, x0 + 1, y0 + 1);
     s.light[0] = bl(a.v0.x, b.v0.x, c.v0.x, d.v0.x, fx, fy);
     s.light[1] = bl(a.v0.y, b.v0.y, c.v0.y, d.v0.y, fx, fy);
@@ -1575,8 +1579,8 @@ VXD void sample_all(const TaaArgs& A, float u, float v, Sample& s) {
 VXD void sample_last(const TaaArgs& A, float u, float v, float* o) {
     int32_t x0, y0;
     float fx, fy;
-    weight(u * (float)A.width - 0.5f, x0, fx);
-    weight(v * (float)A.height - 0.5f, y0, fy);
+    weight(u * (float)A.width - 0.5f, A.width, x0, fx);
+    weight(v * (float)A.height - 0.5f, A.height, y0, fy);
     float t[4][4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
