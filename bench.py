@@ -184,14 +184,16 @@ def bench_large(ctx, pal_d, reps=3):
 
 
 def _prefab_frame(ctx, P, models, vol, reps=10):
-    """One 1920x1080 frame of a loaded prefab: the multi-instance geometry pass
-    over its draws, then 1 shadow + 16 AO + 1 spec DDA rays per hit pixel over
-    that G-buffer against the prefab's rebuilt volume."""
+    """One 1920x1080 frame of a loaded prefab, the frame after its spawn (every
+    entity's previous world matrix is its world matrix): the multi-instance
+    geometry pass over its draws, 1 shadow + 16 AO + 1 spec DDA rays per hit
+    pixel over that G-buffer against the prefab's rebuilt volume, the shading
+    and the temporal accumulation."""
     import torch
     from voxelengine_amd import api, configs, views
     W, H = 1920, 1080
     vd = views.make_view((8.0, 16.0, 6.0), (32.0, 0.0, 32.0), W, H)
-    cmds = P.vox_cmds(models)
+    cmds = P.vox_cmds(models, prev=[P.world(i) for i in range(len(P.entities))])
     g = api.GBuffer(W, H)
     ext = api.GBufferExt(W, H)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
