@@ -103,6 +103,10 @@ struct vx_ctx {
     uint32_t tile_cap;
     void* taa_rec;           // LightTAA: packed per-pixel attributes (64 B each)
     uint32_t taa_cap;
+    int32_t* rb_aabb;        // rebuild: per-instance AABB slots
+    size_t rb_aabb_cap;
+    FootprintOp* rb_ops;     // rebuild: footprint operations
+    size_t rb_ops_cap;
 };
 
 static void profile_clear(vx_ctx* c) {
@@ -248,6 +252,10 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     c->tile_cap = 0;
     c->taa_rec = nullptr;
     c->taa_cap = 0;
+    c->rb_aabb = nullptr;
+    c->rb_aabb_cap = 0;
+    c->rb_ops = nullptr;
+    c->rb_ops_cap = 0;
     hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
     if (me == hipSuccess) me = hipMalloc(&c->march.counters, kCounterWords * sizeof(uint32_t));
     if (me != hipSuccess) {
@@ -272,6 +280,8 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     (void)hipFree(ctx->tile_lists);
     (void)hipFree(ctx->tile_counts);
     (void)hipFree(ctx->taa_rec);
+    (void)hipFree(ctx->rb_aabb);
+    (void)hipFree(ctx->rb_ops);
     profile_clear(ctx);
     delete ctx;
     return VX_OK;
@@ -696,10 +706,30 @@ extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, ui
         aabb_h[6 * i + 0] = v->sx - 1; aabb_h[6 * i + 1] = v->sy - 1; aabb_h[6 * i + 2] = v->sz - 1;
         aabb_h[6 * i + 3] = 0; aabb_h[6 * i + 4] = 0; aabb_h[6 * i + 5] = 0;
     }
-    int32_t* aabb_d = nullptr;
-    FootprintOp* ops_d = nullptr;
-    VX_HIP(hipMalloc(&aabb_d, aabb_h.size() * sizeof(int32_t)));
-    hipError_t e = hipMalloc(&ops_d, ops.size() * sizeof(FootprintOp));
+    // per-context scratch, grown on demand (the rebuild is synchronous)
+    vx_ctx* cx = v->ctx;
+    if (cx->rb_aabb_cap < aabb_h.size() || cx->rb_ops_cap < ops.size()) {
+        VX_HIP(hipDeviceSynchronize());
+        if (cx->rb_aabb_cap < aabb_h.size()) {
+            (void)hipFree(cx->rb_aabb);
+            cx->rb_aabb = nullptr;
+            cx->rb_aabb_cap = 0;
+            const size_t cap = std::max<size_t>(aabb_h.size(), 6 * 256);
+            VX_HIP(hipMalloc(&cx->rb_aabb, cap * sizeof(int32_t)));
+            cx->rb_aabb_cap = cap;
+        }
+        if (cx->rb_ops_cap < ops.size()) {
+            (void)hipFree(cx->rb_ops);
+            cx->rb_ops = nullptr;
+            cx->rb_ops_cap = 0;
+            const size_t cap = std::max<size_t>(ops.size(), 512);
+            VX_HIP(hipMalloc(&cx->rb_ops, cap * sizeof(FootprintOp)));
+            cx->rb_ops_cap = cap;
+        }
+    }
+    int32_t* aabb_d = cx->rb_aabb;
+    FootprintOp* ops_d = cx->rb_ops;
+    hipError_t e = hipSuccess;
     std::vector<FootprintOp> flat;
     std::vector<size_t> first;
     for (auto& b : batches) {
@@ -712,9 +742,10 @@ extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, ui
         e = hipMemcpyAsync(ops_d, flat.data(), flat.size() * sizeof(FootprintOp), hipMemcpyHostToDevice, S(stream));
     OccView V = occ_view(v);
     for (size_t b = 0; b < batches.size() && e == hipSuccess; ++b) {
-        size_t maxvox = 0;
-        for (int q : batches[b]) maxvox = std::max(maxvox, (size_t)ops[q].op.sx * ops[q].op.sy * ops[q].op.sz);
-        uint32_t gx = std::min<uint32_t>(cdiv(maxvox, 256), 4096);
+        size_t maxq = 0;   // quads of four x voxels
+        for (int q : batches[b])
+            maxq = std::max(maxq, (size_t)((ops[q].op.sx + 3) / 4) * ops[q].op.sy * ops[q].op.sz);
+        uint32_t gx = std::min<uint32_t>(cdiv(maxq, 256), 4096);
         for (size_t c0 = 0; c0 < batches[b].size() && e == hipSuccess; c0 += 65535) {
             uint32_t cnt = (uint32_t)std::min<size_t>(65535, batches[b].size() - c0);
             hipLaunchKernelGGL(k_footprint, dim3(gx, cnt), dim3(256), 0, S(stream), v->cells, V,
@@ -751,8 +782,6 @@ extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, ui
     if (e == hipSuccess)
         e = hipMemcpyAsync(aabb_h.data(), aabb_d, aabb_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, S(stream));
     if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
-    (void)hipFree(aabb_d);
-    (void)hipFree(ops_d);
     if (e != hipSuccess) return fail(VX_ERR_DEVICE, "rebuild failed: %s", hipGetErrorString(e));
 
     // Dirty regions, ShadowVoxSystem.cpp:180-190.

@@ -167,32 +167,80 @@ __global__ void k_cells_to_bytes(uint8_t* bytes, const uint64_t* cells, OccView 
 // One footprint loop of ShadowVoxSystem::OnUpdate / OnVoxDestroyed
 // (ShadowVoxSystem.cpp:142-155, 164-177, 28-40) for a batch of operations that
 // commute (same value, or disjoint footprints). blockIdx.y = operation.
+// A thread takes four consecutive x voxels of a row; voxels that land in
+// the same 32-bit cell word share one atomic. The per-voxel arithmetic is the
+// reference's, voxel by voxel.
 __global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, int32_t* aabb, int32_t thr) {
     const FootprintOp op = ops[blockIdx.y];
-    size_t nvox = (size_t)op.sx * op.sy * op.sz;
+    const uint32_t qx = ((uint32_t)op.sx + 3u) >> 2;
+    const uint64_t nq = (uint64_t)qx * (uint32_t)op.sy * (uint32_t)op.sz;
+    const bool narrow = nq <= 0xffffffffull;
+    // rows of whole quads and a 4-byte aligned model: one dword per quad
+    const bool dword = ((uintptr_t)op.model & 3u) == 0 && (op.sx & 3) == 0;
     int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvox; i += (size_t)gridDim.x * blockDim.x) {
-        uint8_t mat = op.model[i];
-        if (mat < thr) continue;
-        int32_t x = (int32_t)(i % op.sx);
-        int32_t y = (int32_t)((i / op.sx) % op.sy);
-        int32_t z = (int32_t)(i / ((size_t)op.sx * op.sy));
-        int32_t f[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float wp = ((op.o[k] + (float)x * op.dx[k]) + (float)y * op.dy[k]) + (float)z * op.dz[k];
-            f[k] = f2i(wp * 10.0f);
-            mn[k] = min(mn[k], f[k]);
-            mx[k] = max(mx[k], f[k]);
+    uint32_t* words = reinterpret_cast<uint32_t*>(cells);
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t m4 = 0;
+        if (dword) {
+            m4 = reinterpret_cast<const uint32_t*>(op.model)[q];
+            if ((int32_t)(m4 & 255u) < thr && (int32_t)((m4 >> 8) & 255u) < thr &&
+                (int32_t)((m4 >> 16) & 255u) < thr && (int32_t)(m4 >> 24) < thr)
+                continue;
         }
-        // SetVolumeAt (ShadowVoxSystem.cpp:82-94): out-of-range writes are dropped.
-        if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] >= 2 * V.sx || f[1] >= 2 * V.sy || f[2] >= 2 * V.sz) continue;
-        size_t ci = cell_index(V, f[0], f[1], f[2]);
-        uint32_t b = cell_bit(f[0], f[1], f[2]);
-        uint32_t* w32 = reinterpret_cast<uint32_t*>(cells + ci) + (b >> 5);
-        uint32_t m = 1u << (b & 31u);
-        if (op.value) atomicOr(w32, m);
-        else atomicAnd(w32, ~m);
+        uint32_t xq, y, z;
+        if (narrow) {
+            const uint32_t q32 = (uint32_t)q, r = q32 / qx;
+            xq = q32 - r * qx;
+            y = r % (uint32_t)op.sy;
+            z = r / (uint32_t)op.sy;
+        } else {
+            const uint64_t r = q / qx;
+            xq = (uint32_t)(q - r * qx);
+            y = (uint32_t)(r % (uint32_t)op.sy);
+            z = (uint32_t)(r / (uint32_t)op.sy);
+        }
+        const uint32_t nvalid = min(4u, (uint32_t)op.sx - 4u * xq);
+        if (!dword) {
+            const uint8_t* p = op.model + (((size_t)z * (uint32_t)op.sy + y) * (uint32_t)op.sx + 4u * xq);
+            for (uint32_t j = 0; j < nvalid; ++j) m4 |= (uint32_t)p[j] << (8 * j);
+        }
+        size_t wa[4];
+        uint32_t wm[4];
+        bool has[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            has[j] = false;
+            wa[j] = 0;
+            wm[j] = 0;
+            if ((uint32_t)j >= nvalid || (int32_t)((m4 >> (8 * j)) & 255u) < thr) continue;
+            const int32_t x = (int32_t)(4 * xq + j);
+            int32_t f[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float wp = ((op.o[k] + (float)x * op.dx[k]) + (float)y * op.dy[k]) + (float)z * op.dz[k];
+                f[k] = f2i(wp * 10.0f);
+                mn[k] = min(mn[k], f[k]);
+                mx[k] = max(mx[k], f[k]);
+            }
+            // SetVolumeAt (ShadowVoxSystem.cpp:82-94): out-of-range writes are dropped.
+            if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] >= 2 * V.sx || f[1] >= 2 * V.sy || f[2] >= 2 * V.sz) continue;
+            const uint32_t bit = cell_bit(f[0], f[1], f[2]);
+            wa[j] = cell_index(V, f[0], f[1], f[2]) * 2 + (bit >> 5);
+            wm[j] = 1u << (bit & 31u);
+            has[j] = true;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!has[j]) continue;
+#pragma unroll
+            for (int k = j + 1; k < 4; ++k)
+                if (has[k] && wa[k] == wa[j]) {
+                    wm[j] |= wm[k];
+                    has[k] = false;
+                }
+            if (op.value) atomicOr(words + wa[j], wm[j]);
+            else atomicAnd(words + wa[j], ~wm[j]);
+        }
     }
     // AABB of the footprint (ShadowVoxSystem.cpp:148-149): wave reduction, one
     // atomic per wave into the instance's slot.
