@@ -1074,19 +1074,32 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
     uint32_t lit = 1, nf = 0;
     float aosum = 0.0f, spv = 1.0f;
     const size_t base = (((size_t)gt * (size_t)A.nrays) << 6) | l;
-    for (int32_t r = 0; r < A.nrays; ++r) {
-        const size_t s = base + ((size_t)r << 6);
-        const float v = A.res[s];
-        const bool hit = __float_as_uint(v) != kNoHitBits;
-        if (r < P.n_shadow) {
-            lit = hit ? 0u : 1u;
-        } else {
-            const float maxt = r < P.n_shadow + P.n_ao ? P.ao_maxt : P.spec_maxt;
-            const float q = ray_value(RAY_AO, hit ? 1 : 0, v, maxt);
-            if (r < P.n_shadow + P.n_ao) aosum = aosum + q * q;
-            else spv = q;
+    // eight slots in flight per pass, folded in ray order
+    for (int32_t r0 = 0; r0 < A.nrays; r0 += 8) {
+        float vv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            vv[k] = r0 + k < A.nrays ? A.res[base + ((size_t)(r0 + k) << 6)] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t r = r0 + k;
+            if (r >= A.nrays) break;
+            const float v = vv[k];
+            const bool hit = __float_as_uint(v) != kNoHitBits;
+            if (r < P.n_shadow) {
+                lit = hit ? 0u : 1u;
+            } else {
+                const float maxt = r < P.n_shadow + P.n_ao ? P.ao_maxt : P.spec_maxt;
+                const float q = ray_value(RAY_AO, hit ? 1 : 0, v, maxt);
+                if (r < P.n_shadow + P.n_ao) aosum = aosum + q * q;
+                else spv = q;
+            }
         }
-        if (A.fres) nf += A.fres[s];
+        if (A.fres) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (r0 + k < A.nrays) nf += A.fres[base + ((size_t)(r0 + k) << 6)];
+        }
     }
     T.shadow[i] = (uint8_t)lit;
     T.ao[i] = P.n_ao > 0 ? aosum / (float)P.n_ao : 1.0f;
