@@ -1281,27 +1281,32 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
 // Primary hit: GeometryVoxel.frag (LOD 1, MIP_COUNT 2, TRANSPARENCY 1) for the
 // world model with an identity WorldMatrix.
 // ===========================================================================
-VXD uint32_t mip_fetch(const ModelMips& A, int32_t mip, int32_t x, int32_t y, int32_t z) {
-    int32_t mx = A.dims[0] >> mip, my = A.dims[1] >> mip, mz = A.dims[2] >> mip;
-    if ((uint32_t)x >= (uint32_t)mx || (uint32_t)y >= (uint32_t)my || (uint32_t)z >= (uint32_t)mz) return 0u;
-    return A.mips[mip][(size_t)x + (size_t)y * mx + (size_t)z * mx * my];
-}
-
 // intersectVolume, GeometryVoxel.frag:63-125, in the model's voxel space.
 // res / frame: the view's resolution and frame (the glass dither).
+// The select products step(a,b)*step(a,c) are 0 or 1 and sign(d) is -1, 0 or
+// 1, so the voxel step ivec3(select * stepSign) is taken as an integer select;
+// t_max, best_t and the normal keep the reference's float products (an
+// infinite t_delta times a 0 select is NaN there too).
 VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t frame, f3 o3, f3 d3, f3 cam,
                              float uvx, float uvy, f3& hit, f3& nrm, uint32_t& material) {
     float o[3] = {o3.x, o3.y, o3.z}, d[3] = {d3.x, d3.y, d3.z}, c[3] = {cam.x, cam.y, cam.z};
     float s[3], td[3];
+    int32_t is[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         s[k] = gsign(d[k]);
         td[k] = 1.0f / (d[k] * s[k]);
+        is[k] = (int32_t)s[k];
     }
+    // voxel indices fit 32 bits unless mip 0 holds 2^32 voxels or more
+    const bool narrow = (uint64_t)(uint32_t)A.dims[0] * (uint32_t)A.dims[1] * (uint32_t)A.dims[2] < (1ull << 32);
     int32_t mip = 2;
     int32_t i = 0;
     do {
         float mip_size = (float)(1 << mip);
+        const uint8_t* base = mip == 2 ? A.mips[2] : (mip == 1 ? A.mips[1] : A.mips[0]);
+        const uint32_t mx = (uint32_t)(A.dims[0] >> mip), my = (uint32_t)(A.dims[1] >> mip),
+                       mz = (uint32_t)(A.dims[2] >> mip);
         int32_t cv[3], up[3];
         float tm[3];
 #pragma unroll
@@ -1314,15 +1319,22 @@ VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t fra
         }
         int32_t n = 0;
         do {
-            float s0 = gstep(tm[0], tm[2]) * gstep(tm[0], tm[1]);
-            float s1 = gstep(tm[1], tm[0]) * gstep(tm[1], tm[2]);
-            float s2 = gstep(tm[2], tm[1]) * gstep(tm[2], tm[0]);
-            cv[0] += f2i(s0 * s[0]);
-            cv[1] += f2i(s1 * s[1]);
-            cv[2] += f2i(s2 * s[2]);
+            const bool b0 = !(tm[2] < tm[0]) && !(tm[1] < tm[0]);
+            const bool b1 = !(tm[0] < tm[1]) && !(tm[2] < tm[1]);
+            const bool b2 = !(tm[1] < tm[2]) && !(tm[0] < tm[2]);
+            const float s0 = b0 ? 1.0f : 0.0f, s1 = b1 ? 1.0f : 0.0f, s2 = b2 ? 1.0f : 0.0f;
+            cv[0] += b0 ? is[0] : 0;
+            cv[1] += b1 ? is[1] : 0;
+            cv[2] += b2 ? is[2] : 0;
             if (cv[0] < -1 || cv[0] > up[0] || cv[1] < -1 || cv[1] > up[1] || cv[2] < -1 || cv[2] > up[2])
                 return 0;
-            uint32_t vox = mip_fetch(A, mip, cv[0], cv[1], cv[2]);
+            uint32_t vox = 0u;
+            if ((uint32_t)cv[0] < mx && (uint32_t)cv[1] < my && (uint32_t)cv[2] < mz) {
+                if (narrow)
+                    vox = base[(uint32_t)cv[0] + (uint32_t)cv[1] * mx + (uint32_t)cv[2] * (mx * my)];
+                else
+                    vox = base[(size_t)cv[0] + (size_t)cv[1] * mx + (size_t)cv[2] * mx * my];
+            }
             if (vox != 0u) {
                 float best_t = (tm[0] * s0 + tm[1] * s1) + tm[2] * s2;
                 bool accept = (mip == 0);
