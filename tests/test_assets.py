@@ -219,3 +219,13 @@ def test_prefab_errors(tmp_path):
     assert P.entities[0].model == -1 and P.entities[0].vox_guid == 0xABC
     assert P.rebuild_order() == [0]
     assert P.instances({}) == []
+
+
+def test_modernhouse_model_counts_match_the_asset(mods):
+    """ModernHouse/1.v is 160x88x204 with 226,069 non-zero voxels, 219,622 of
+    them >= 16 (the rebuild's occupancy threshold), read through vx_vox_parse."""
+    data = open(os.path.join(mods, "default", "ModernHouse", "1.v"), "rb").read()
+    dims, vox = api.parse_vox(data)
+    assert dims == (160, 88, 204)
+    assert int((vox > 0).sum()) == 226069
+    assert int((vox >= 16).sum()) == 219622
