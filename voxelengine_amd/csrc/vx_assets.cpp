@@ -17,7 +17,9 @@
 #include <cstring>
 #include <dirent.h>
 #include <map>
+#include <exception>
 #include <memory>
+#include <new>
 #include <string>
 #include <sys/stat.h>
 #include <unordered_map>
@@ -36,6 +38,20 @@ vx_status failf(vx_status code, const char* fmt, ...) {
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
     return vx_set_error(code, buf);
+}
+
+// Runs an entry point's body; C++ exceptions do not cross the C ABI.
+template <class F>
+vx_status guarded(const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return failf(VX_ERR_OUT_OF_MEMORY, "%s: host allocation failed", what);
+    } catch (const std::exception& e) {
+        return failf(VX_ERR_INVALID_ARGUMENT, "%s: %s", what, e.what());
+    } catch (...) {
+        return failf(VX_ERR_INVALID_ARGUMENT, "%s failed", what);
+    }
 }
 
 bool read_file(const std::string& path, std::vector<uint8_t>& out) {
@@ -391,23 +407,25 @@ extern "C" vx_status vx_vox_parse(const uint8_t* bytes, size_t nbytes, int32_t d
 extern "C" vx_status vx_model_load(vx_ctx* ctx, const char* path, vx_model** out, void* stream) {
     if (!ctx || !path || !out) return failf(VX_ERR_INVALID_ARGUMENT, "vx_model_load: NULL argument");
     *out = nullptr;
-    std::vector<uint8_t> b;
-    if (!read_file(path, b)) return failf(VX_ERR_INVALID_ARGUMENT, "cannot read %s", path);
-    int32_t d[3];
-    size_t off = 0;
-    vx_status st = vx_vox_parse(b.data(), b.size(), d, &off);
-    if (st != VX_OK) return st;
-    if ((d[0] | d[1] | d[2]) & 3)
-        return failf(VX_ERR_UNSUPPORTED, "%s: sizes %dx%dx%d are not multiples of 4", path, d[0], d[1], d[2]);
-    vx_model* m = nullptr;
-    if ((st = vx_model_create(ctx, d[0], d[1], d[2], &m)) != VX_OK) return st;
-    const size_t n = (size_t)d[0] * d[1] * d[2];
-    if ((st = vx_model_upload(m, b.data() + off, n, stream)) != VX_OK || (st = vx_model_build_mips(m, stream)) != VX_OK) {
-        vx_model_destroy(m);
-        return st;
-    }
-    *out = m;
-    return VX_OK;
+    return guarded("vx_model_load", [&]() -> vx_status {
+        std::vector<uint8_t> b;
+        if (!read_file(path, b)) return failf(VX_ERR_INVALID_ARGUMENT, "cannot read %s", path);
+        int32_t d[3];
+        size_t off = 0;
+        vx_status st = vx_vox_parse(b.data(), b.size(), d, &off);
+        if (st != VX_OK) return st;
+        if ((d[0] | d[1] | d[2]) & 3)
+            return failf(VX_ERR_UNSUPPORTED, "%s: sizes %dx%dx%d are not multiples of 4", path, d[0], d[1], d[2]);
+        vx_model* m = nullptr;
+        if ((st = vx_model_create(ctx, d[0], d[1], d[2], &m)) != VX_OK) return st;
+        const size_t n = (size_t)d[0] * d[1] * d[2];
+        if ((st = vx_model_upload(m, b.data() + off, n, stream)) != VX_OK || (st = vx_model_build_mips(m, stream)) != VX_OK) {
+            vx_model_destroy(m);
+            return st;
+        }
+        *out = m;
+        return VX_OK;
+    });
 }
 
 extern "C" vx_status vx_palette_parse(const uint8_t* bytes, size_t nbytes, float* palette) {
@@ -423,9 +441,11 @@ extern "C" vx_status vx_palette_parse(const uint8_t* bytes, size_t nbytes, float
 
 extern "C" vx_status vx_palette_load(const char* path, float* palette) {
     if (!path || !palette) return failf(VX_ERR_INVALID_ARGUMENT, "vx_palette_load: NULL argument");
-    std::vector<uint8_t> b;
-    if (!read_file(path, b)) return failf(VX_ERR_INVALID_ARGUMENT, "cannot read %s", path);
-    return vx_palette_parse(b.data(), b.size(), palette);
+    return guarded("vx_palette_load", [&]() -> vx_status {
+        std::vector<uint8_t> b;
+        if (!read_file(path, b)) return failf(VX_ERR_INVALID_ARGUMENT, "cannot read %s", path);
+        return vx_palette_parse(b.data(), b.size(), palette);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -562,40 +582,42 @@ int32_t spawn(vx_prefab* P, const std::string& rel, int32_t parent_entity, int d
 extern "C" vx_status vx_prefab_load(const char* mods_dir, const char* path, vx_prefab** out) {
     if (!mods_dir || !path || !out) return failf(VX_ERR_INVALID_ARGUMENT, "vx_prefab_load: NULL argument");
     *out = nullptr;
-    std::unique_ptr<vx_prefab> P(new vx_prefab());
-    P->mods_dir = mods_dir;
-    scan_dir(P->mods_dir, "", P->assets, 0);
-    try {
-        spawn(P.get(), path, -1, 0);
-    } catch (const SpawnErr& e) {
-        return failf(VX_ERR_INVALID_ARGUMENT, "%s", e.msg.c_str());
-    }
-    // TransformSystem::OnUpdate (TransformSystem.cpp:36-70): entities in
-    // creation order (the Dirty sort by entity id); RealculateMatrix
-    // (TransformSystem.cpp:124-135) with the parent's world matrix, or the
-    // identity for an entity without a parent transform.
-    for (size_t i = 0; i < P->ents.size(); ++i) {
-        vx_prefab_entity& x = P->ents[i];
-        if (!(x.flags & VX_ENT_TRANSFORM)) {
-            // World::Create adds no Transform; such entities keep identity matrices
-            mat_identity(x.matrix);
-            mat_identity(x.world);
-            continue;
+    return guarded("vx_prefab_load", [&]() -> vx_status {
+        std::unique_ptr<vx_prefab> P(new vx_prefab());
+        P->mods_dir = mods_dir;
+        scan_dir(P->mods_dir, "", P->assets, 0);
+        try {
+            spawn(P.get(), path, -1, 0);
+        } catch (const SpawnErr& e) {
+            return failf(VX_ERR_INVALID_ARGUMENT, "%s", e.msg.c_str());
         }
-        static const float kZ[3] = {0, 0, 1}, kY[3] = {0, 1, 0}, kX[3] = {1, 0, 0};
-        mat_identity(x.matrix);
-        mat_translate(x.matrix, x.position);
-        mat_rotate(x.matrix, x.rotation[2], kZ);
-        mat_rotate(x.matrix, x.rotation[1], kY);
-        mat_rotate(x.matrix, x.rotation[0], kX);
-        mat_scale(x.matrix, x.scale);
-        float parent[16];
-        if (x.parent >= 0 && (P->ents[x.parent].flags & VX_ENT_TRANSFORM)) memcpy(parent, P->ents[x.parent].world, sizeof(parent));
-        else mat_identity(parent);
-        mat_mul(parent, x.matrix, x.world);
-    }
-    *out = P.release();
-    return VX_OK;
+        // TransformSystem::OnUpdate (TransformSystem.cpp:36-70): entities in
+        // creation order (the Dirty sort by entity id); RealculateMatrix
+        // (TransformSystem.cpp:124-135) with the parent's world matrix, or the
+        // identity for an entity without a parent transform.
+        for (size_t i = 0; i < P->ents.size(); ++i) {
+            vx_prefab_entity& x = P->ents[i];
+            if (!(x.flags & VX_ENT_TRANSFORM)) {
+                // World::Create adds no Transform; such entities keep identity matrices
+                mat_identity(x.matrix);
+                mat_identity(x.world);
+                continue;
+            }
+            static const float kZ[3] = {0, 0, 1}, kY[3] = {0, 1, 0}, kX[3] = {1, 0, 0};
+            mat_identity(x.matrix);
+            mat_translate(x.matrix, x.position);
+            mat_rotate(x.matrix, x.rotation[2], kZ);
+            mat_rotate(x.matrix, x.rotation[1], kY);
+            mat_rotate(x.matrix, x.rotation[0], kX);
+            mat_scale(x.matrix, x.scale);
+            float parent[16];
+            if (x.parent >= 0 && (P->ents[x.parent].flags & VX_ENT_TRANSFORM)) memcpy(parent, P->ents[x.parent].world, sizeof(parent));
+            else mat_identity(parent);
+            mat_mul(parent, x.matrix, x.world);
+        }
+        *out = P.release();
+        return VX_OK;
+    });
 }
 
 extern "C" vx_status vx_prefab_destroy(vx_prefab* p) {
