@@ -34,6 +34,20 @@ static vx_status fail(vx_status code, const char* fmt, ...) {
     return code;
 }
 
+// Runs an entry point's body; C++ exceptions (host allocation) do not cross the C ABI.
+template <class F>
+static vx_status guarded(const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(VX_ERR_OUT_OF_MEMORY, "%s: host allocation failed", what);
+    } catch (const std::exception& e) {
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s: %s", what, e.what());
+    } catch (...) {
+        return fail(VX_ERR_INVALID_ARGUMENT, "%s failed", what);
+    }
+}
+
 // The host-only sources (asset loaders) report through the same per-thread message.
 vx_status vx_set_error(vx_status code, const char* msg) {
     g_last_error = msg;
@@ -643,7 +657,7 @@ static bool disjoint(const PlannedOp& a, const PlannedOp& b) {
     return false;
 }
 
-extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, uint32_t n, int32_t thr,
+static vx_status vx_volume_rebuild_impl(vx_volume* v, const vx_instance* inst, uint32_t n, int32_t thr,
                                        vx_region* dirty, uint32_t* ndirty, void* stream) {
     VX_REQUIRE(v, "volume is NULL");
     VX_REQUIRE(n == 0 || inst, "instances is NULL");
@@ -806,7 +820,7 @@ extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, ui
     return VX_OK;
 }
 
-extern "C" vx_status vx_volume_build_levels(vx_volume* v, const vx_region* regions, uint32_t nregions, void* stream) {
+static vx_status vx_volume_build_levels_impl(vx_volume* v, const vx_region* regions, uint32_t nregions, void* stream) {
     VX_REQUIRE(v, "volume is NULL");
     VX_REQUIRE(nregions == 0 || regions, "regions is NULL");
     if (v->nlevels == 0) return VX_OK;
@@ -984,7 +998,7 @@ static void inst_rect(const vx_view& v, const float* world, const int32_t dims[3
     r[2] = (int32_t)std::min(x1, (double)(v.width - 1)); r[3] = (int32_t)std::min(y1, (double)(v.height - 1));
 }
 
-extern "C" vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, const vx_view* view,
+static vx_status vx_primary_instances_impl(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, const vx_view* view,
                                           const vx_gbuf* g, const vx_gbuf_ext* ext, void* stream) {
     VX_REQUIRE(ctx && g, "ctx/gbuf is NULL");
     if (vx_status st = check_view(view)) return st;
@@ -1340,7 +1354,7 @@ extern "C" vx_status vx_shade_backward(vx_ctx* ctx, const vx_view* view, const v
     return VX_OK;
 }
 
-extern "C" vx_status vx_shade_backward_batch(vx_ctx* ctx, const vx_shade_item* items, uint32_t n,
+static vx_status vx_shade_backward_batch_impl(vx_ctx* ctx, const vx_shade_item* items, uint32_t n,
                                              const float* palette, const float light[6], float* dpalette,
                                              float* dlight, void* stream) {
     VX_REQUIRE(ctx && dpalette && dlight, "ctx/dpalette/dlight is NULL");
@@ -1419,7 +1433,7 @@ static hipError_t batch_enqueue(vx_batch* b, hipStream_t s, hipEvent_t e0 = null
                         e1);
 }
 
-extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
+static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
                                      const vx_trace_params* params, vx_batch** out) {
     VX_REQUIRE(ctx && out && palette, "ctx/out/palette is NULL");
     VX_REQUIRE(n >= 1 && n <= 65536 && items, "batch needs 1..65536 items");
@@ -1512,7 +1526,7 @@ extern "C" vx_status vx_batch_launch(vx_batch* b, void* stream) {
     return VX_OK;
 }
 
-extern "C" vx_status vx_batch_launch_direct(vx_batch* b, void* stream) {
+static vx_status vx_batch_launch_direct_impl(vx_batch* b, void* stream) {
     VX_REQUIRE(b, "batch is NULL");
     if (vx_status st = use_device(b->ctx)) return st;
     vx_ctx* ctx = b->ctx;
@@ -1536,4 +1550,41 @@ extern "C" vx_status vx_batch_destroy(vx_batch* b) {
     (void)hipDeviceSynchronize();
     batch_free(b);
     return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// entry points whose bodies allocate host containers
+// ---------------------------------------------------------------------------
+extern "C" vx_status vx_volume_rebuild(vx_volume* v, const vx_instance* inst, uint32_t n, int32_t thr,
+                                       vx_region* dirty, uint32_t* ndirty, void* stream) {
+    return guarded("vx_volume_rebuild",
+                   [&] { return vx_volume_rebuild_impl(v, inst, n, thr, dirty, ndirty, stream); });
+}
+
+extern "C" vx_status vx_volume_build_levels(vx_volume* v, const vx_region* regions, uint32_t nregions, void* stream) {
+    return guarded("vx_volume_build_levels",
+                   [&] { return vx_volume_build_levels_impl(v, regions, nregions, stream); });
+}
+
+extern "C" vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, const vx_view* view,
+                                          const vx_gbuf* g, const vx_gbuf_ext* ext, void* stream) {
+    return guarded("vx_primary_instances",
+                   [&] { return vx_primary_instances_impl(ctx, cmds, n, view, g, ext, stream); });
+}
+
+extern "C" vx_status vx_shade_backward_batch(vx_ctx* ctx, const vx_shade_item* items, uint32_t n,
+                                             const float* palette, const float light[6], float* dpalette,
+                                             float* dlight, void* stream) {
+    return guarded("vx_shade_backward_batch", [&] {
+        return vx_shade_backward_batch_impl(ctx, items, n, palette, light, dpalette, dlight, stream);
+    });
+}
+
+extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
+                                     const vx_trace_params* params, vx_batch** out) {
+    return guarded("vx_batch_create", [&] { return vx_batch_create_impl(ctx, items, n, palette, params, out); });
+}
+
+extern "C" vx_status vx_batch_launch_direct(vx_batch* b, void* stream) {
+    return guarded("vx_batch_launch_direct", [&] { return vx_batch_launch_direct_impl(b, stream); });
 }
