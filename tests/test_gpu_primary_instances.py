@@ -147,3 +147,37 @@ def test_farmhouse_draws_feed_the_trace(ctx, oracle, tmp_path):
     assert_bitwise(vis.shadow.cpu().numpy(), sh, "shadow")
     assert_bitwise(vis.ao.cpu().numpy(), ao, "ao")
     assert 0.05 < (sh[ref["mat"] > 0] > 0).mean() < 0.98
+
+
+def test_geometry_pass_error_paths(ctx):
+    """Limits and bad input come back as errors before any launch."""
+    import torch
+    from voxelengine_amd import api, _lib as L
+    m = api.Model(ctx, 8, 8, 8)
+    g = api.GBuffer(16, 16)
+    vd = views.make_view((0.0, 2.0, 0.0), (1.0, 0.0, 1.0), 16, 16)
+    eye = np.eye(4, dtype=np.float32).reshape(16)
+    with pytest.raises(L.VXError, match="at most 4096"):
+        api.primary_instances(ctx, [api.make_vox_cmd(m, eye)] * 4097, vd, g)
+    with pytest.raises(L.VXError, match="singular"):
+        api.primary_instances(ctx, [api.make_vox_cmd(m, np.zeros(16, np.float32))], vd, g)
+    with pytest.raises(L.VXError, match="palette"):
+        api.primary_instances(ctx, [api.make_vox_cmd(m, eye, palette=70000)], vd, g)
+    api.primary_instances(ctx, [], vd, g)           # no draws: every pixel misses
+    torch.cuda.synchronize()
+    assert int(g.material.sum().item()) == 0
+
+
+def test_taa_error_paths(ctx):
+    import torch
+    from voxelengine_amd import api, configs, _lib as L
+    W, H = 16, 16
+    vd = views.make_view((0.0, 2.0, 0.0), (1.0, 0.0, 1.0), W, H)
+    z = lambda *s, dt=torch.float32: torch.zeros(s, dtype=dt, device="cuda")
+    pal = torch.from_numpy(configs.make_palette()).cuda()
+    last = z(W * H, 4)
+    with pytest.raises(L.VXError, match="alias"):
+        api.light_taa(ctx, vd, z(W * H, 3), last, last, z(W * H), z(W * H, 3), z(W * H, dt=torch.uint8), pal)
+    with pytest.raises(L.VXError, match="aligned"):
+        api.light_taa(ctx, vd, z(W * H, 3), z(W * H * 4 + 1)[1:], z(W * H, 4), z(W * H), z(W * H, 3),
+                      z(W * H, dt=torch.uint8), pal)
