@@ -1057,7 +1057,11 @@ __global__ void __launch_bounds__(256) k_march_rest(MarchArgs A) {
 // Per-pixel fold of the ray slots in ray order (LightAmbient.frag:160-175:
 // shadow lit, mean of the squared AO values, spec value) and sky pixels.
 template <bool SINGLE>
-VXD void resolve_tile(const MarchArgs& A, const MarchItem& I, uint32_t gt, uint32_t l) {
+__global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
+    const uint32_t gt = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (gt >= A.ntiles) return;
+    const uint32_t l = threadIdx.x & 63u;
+    const MarchItem& I = march_item<SINGLE>(A, __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gt)));
     uint32_t i = 0;
     if (!tile_lane_pixel(I, gt - I.tile0, l, i)) return;
     const TraceArgs& T = I.T;
@@ -1101,25 +1105,6 @@ VXD void resolve_tile(const MarchArgs& A, const MarchItem& I, uint32_t gt, uint3
     T.ao[i] = P.n_ao > 0 ? aosum / (float)P.n_ao : 1.0f;
     T.spec[i] = P.n_spec > 0 ? spv : 1.0f;
     if (T.fetches) T.fetches[i] = nf;
-}
-
-// One wave per run of consecutive tiles: the item (found by binary search over
-// the items' first tiles) is looked up again only when the run enters the next one.
-template <bool SINGLE>
-__global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
-    const uint32_t l = threadIdx.x & 63u;
-    const uint32_t nw = gridDim.x * 4u, w = blockIdx.x * 4u + (threadIdx.x >> 6);
-    const uint32_t chunk = (A.ntiles + nw - 1u) / nw;
-    const uint32_t t0 = w * chunk, t1 = min(t0 + chunk, A.ntiles);
-    uint32_t k = 0, lo = 1u, hi = 0u;
-    for (uint32_t gt = t0; gt < t1; ++gt) {
-        if (!SINGLE && !(gt >= lo && gt < hi)) {
-            k = __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gt));
-            lo = A.items[k].tile0;
-            hi = lo + A.items[k].ntiles;
-        }
-        resolve_tile<SINGLE>(A, march_item<SINGLE>(A, k), gt, l);
-    }
 }
 
 // Launchers (template instances live in this file).
@@ -1167,7 +1152,7 @@ hipError_t launch_march(int block, int steps, bool single, uint32_t grid, uint32
         e = hipEventRecord(e1, s);
         if (e != hipSuccess) return e;
     }
-    uint32_t rgrid = std::min((A.ntiles + 3u) / 4u, 8192u);
+    uint32_t rgrid = (A.ntiles + 3u) / 4u;
     if (single) {
         hipLaunchKernelGGL(k_march_rest<true>, dim3(rest_grid), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_resolve<true>, dim3(rgrid), dim3(256), 0, s, A);
