@@ -1404,6 +1404,7 @@ struct vx_batch {
     vx_ctx* ctx;
     uint32_t n;
     MarchItem* items;            // device
+    uint16_t* tile_item;         // device: item of each global tile
     MarchBuffers march;
     MarchArgs args;
     MarchShape shape;
@@ -1419,6 +1420,7 @@ static void batch_free(vx_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap) (void)hipStreamDestroy(b->cap);
     (void)hipFree(b->items);
+    (void)hipFree(b->tile_item);
     march_buffers_free(b->march);
     delete b;
 }
@@ -1474,8 +1476,14 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
     std::memset(b, 0, sizeof(*b));
     b->ctx = ctx;
     b->n = n;
+    std::vector<uint16_t> tile_item((size_t)tiles);
+    for (uint32_t i = 0; i < n; ++i)
+        std::fill(tile_item.begin() + h[i].tile0, tile_item.begin() + h[i].tile0 + h[i].ntiles, (uint16_t)i);
     hipError_t e = hipMalloc(&b->items, n * sizeof(MarchItem));
     if (e == hipSuccess) e = hipMemcpy(b->items, h.data(), n * sizeof(MarchItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess && tiles) e = hipMalloc(&b->tile_item, tile_item.size() * sizeof(uint16_t));
+    if (e == hipSuccess && tiles)
+        e = hipMemcpy(b->tile_item, tile_item.data(), tile_item.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         batch_free(b);
         return fail(VX_ERR_OUT_OF_MEMORY, "batch allocation failed: %s", hipGetErrorString(e));
@@ -1489,6 +1497,7 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
     A.item0 = h[0];
     march_common(A, *params, ctx->tuning);
     A.items = b->items;
+    A.tile_item = b->tile_item;
     A.nitems = n;
     A.ntiles = (uint32_t)tiles;
     A.band = cdiv(tiles, 8);

@@ -55,6 +55,7 @@ constexpr uint32_t kCounterWords = 9 * kCounterStride;
 struct MarchArgs {
     MarchItem item0;           // the item of a single-view launch (a copy of items[0] otherwise)
     const MarchItem* items;    // device array of a batched launch
+    const uint16_t* tile_item; // item of each global tile (batched launches; null: search)
     uint32_t nitems, ntiles;   // items, global tiles
     uint32_t band;             // tiles per XCD band
     uint32_t* counters;        // band counters, listed-ray count (kCounterStride apart)

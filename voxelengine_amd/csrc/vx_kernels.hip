@@ -771,6 +771,7 @@ VXD const MarchItem& march_item(const MarchArgs& A, uint32_t k) {
 template <bool SINGLE>
 VXD uint32_t item_of_tile(const MarchArgs& A, uint32_t gt) {
     if (SINGLE) return 0;
+    if (A.tile_item) return A.tile_item[gt];
     uint32_t lo = 0, hi = A.nitems - 1;
     while (lo < hi) {
         uint32_t mid = (lo + hi + 1u) >> 1;
