@@ -1062,6 +1062,12 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
     const uint32_t gt = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (gt >= A.ntiles) return;
     const uint32_t l = threadIdx.x & 63u;
+    // the tile's slots exist for all 64 lanes: the first eight are loaded
+    // while the item, the pixel and its material are looked up
+    const size_t base = (((size_t)gt * (size_t)A.nrays) << 6) | l;
+    float first[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) first[k] = k < A.nrays ? A.res[base + ((size_t)k << 6)] : 0.0f;
     const MarchItem& I = march_item<SINGLE>(A, __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gt)));
     uint32_t i = 0;
     if (!tile_lane_pixel(I, gt - I.tile0, l, i)) return;
@@ -1074,13 +1080,12 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
     }
     uint32_t lit = 1, nf = 0;
     float aosum = 0.0f, spv = 1.0f;
-    const size_t base = (((size_t)gt * (size_t)A.nrays) << 6) | l;
     // eight slots in flight per pass, folded in ray order
     for (int32_t r0 = 0; r0 < A.nrays; r0 += 8) {
         float vv[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            vv[k] = r0 + k < A.nrays ? A.res[base + ((size_t)(r0 + k) << 6)] : 0.0f;
+            vv[k] = r0 == 0 ? first[k] : (r0 + k < A.nrays ? A.res[base + ((size_t)(r0 + k) << 6)] : 0.0f);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int32_t r = r0 + k;
