@@ -629,7 +629,7 @@ VXD void trace_pixel_exact(const OccView& V, const TraceArgs& A, size_t i) {
 // ===========================================================================
 
 constexpr int kRing = 128;       // ring entries per wave
-constexpr int kRingFields = 11;  // t_max[3], t_delta[3], x|y<<16, z|signs<<16|kind<<19, slot, item, grid word
+constexpr int kRingFields = 10;  // t_max[3], t_delta[3], x|y<<16, z|signs<<16|kind<<19, slot, grid word
 
 VXD bool fast_ray_ok(f3 o, f3 d) {
     const float lo = 7.8886091e-31f;   // 2^-100
@@ -767,6 +767,25 @@ VXD const MarchItem& march_item(const MarchArgs& A, uint32_t k) {
     return A.items[k];
 }
 
+// The same for a wave-uniform item: the items are read-only for the whole
+// launch, so they are read through the constant address space, which keeps
+// the item's fields in scalar registers.
+template <bool SINGLE, std::enable_if_t<SINGLE, int> = 0>
+VXD const MarchItem& march_item_uniform(const MarchArgs& A, uint32_t) {
+    return A.item0;
+}
+template <bool SINGLE, std::enable_if_t<!SINGLE, int> = 0>
+VXD MarchItem march_item_uniform(const MarchArgs& A, uint32_t k) {
+    static_assert(sizeof(MarchItem) % 4 == 0, "MarchItem is whole words");
+    typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+    const ConstWords src = (ConstWords)(A.items + __builtin_amdgcn_readfirstlane(k));
+    MarchItem I;
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&I);
+#pragma unroll
+    for (uint32_t w = 0; w < sizeof(MarchItem) / 4; ++w) dst[w] = src[w];
+    return I;
+}
+
 // Item of global tile gt (items ordered by tile0).
 template <bool SINGLE>
 VXD uint32_t item_of_tile(const MarchArgs& A, uint32_t gt) {
@@ -798,7 +817,7 @@ VXD void ray_origin_voxel(f3 o, int32_t cv[3]) {
 
 // Writes the ring entry of a fast-path ray that starts inside the bounds.
 VXD void ring_put_ray(uint32_t (*ring)[kRing], uint32_t e, f3 o, f3 d, const int32_t cv[3], int32_t kind,
-                      uint32_t slot, uint32_t item, uint32_t gx, uint32_t gxy) {
+                      uint32_t slot, uint32_t gx, uint32_t gxy) {
     float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
     uint32_t signs = 0;
 #pragma unroll
@@ -813,13 +832,12 @@ VXD void ring_put_ray(uint32_t (*ring)[kRing], uint32_t e, f3 o, f3 d, const int
     ring[6][e] = (uint32_t)(cv[0] + 4) | ((uint32_t)(cv[1] + 4) << 16);
     ring[7][e] = (uint32_t)(cv[2] + 4) | (signs << 16) | ((uint32_t)kind << 19);
     ring[8][e] = slot;
-    ring[9][e] = item;
-    ring[10][e] = __umul24((uint32_t)((cv[2] + 4) >> 2), gxy) +
+    ring[9][e] = __umul24((uint32_t)((cv[2] + 4) >> 2), gxy) +
                   (__umul24((uint32_t)((cv[1] + 4) >> 2), gx) + (uint32_t)((cv[0] + 4) >> 2));
 }
 
-// The ray a lane marches (coordinates as below); meta = n << 20 | kind << 16 |
-// item, so n++ is meta += 1 << 20 and n < 256 is meta < 256 << 20.
+// The ray a lane marches (coordinates as below); meta = (n + 0x700) << 20 |
+// kind << 16, so n++ is meta += 1 << 20 and n < 256 is the sign bit clear.
 struct MarchRay {
     float tm0, tm1, tm2, td0, td1, td2;
     int32_t x, y, z, sx, sy, sz;
@@ -849,10 +867,12 @@ VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gx
     return __umul24((uint32_t)(z >> 6), gxy) + (__umul24((uint32_t)(y >> 4), gx) + (uint32_t)(x >> 2));
 }
 
-// Occupancy: 7 waves per SIMD for a single view (72 VGPRs, 140 KB of LDS rings
-// per CU), 6 for batches, whose lanes also hold per-item grid addresses.
+// Occupancy: 7 waves per SIMD (72 VGPRs, 140 KB of LDS rings per CU). A wave
+// marches one item at a time, so the item's grid address and pitches stay
+// wave-uniform: in a batch, a wave that draws a tile of another item first
+// lets its rays in flight finish, then switches.
 template <int BLOCK, int STEPS, bool SINGLE>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SINGLE ? 7 : 5, 8)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)))
 k_march(MarchArgs A) {
     constexpr int WAVES = BLOCK / 64;
     __shared__ uint32_t ring_s[WAVES][kRingFields][kRing];
@@ -876,6 +896,9 @@ k_march(MarchArgs A) {
     R.tm0 = R.tm1 = R.tm2 = R.td0 = R.td1 = R.td2 = 0.0f;
     R.x = R.y = R.z = 4; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.ci = 0; R.cw = 0;
     bool hasray = false, running = false, stopped = false;
+    // the wave's item (its rays in flight all belong to it)
+    uint32_t citem = SINGLE ? 0u : ~0u, nitem = 0;
+    bool pending = false;
     const uint64_t* grid = A.item0.grid;
     uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy;
 
@@ -884,8 +907,8 @@ k_march(MarchArgs A) {
         if (runm == 0 || (uint32_t)__popcll(~runm) >= (uint32_t)A.refill) {
             // ---- finish ended rays
             if (hasray && !running) {
-                const uint32_t kind = (R.meta >> 16) & 3u, n = R.meta >> 20;
-                const MarchItem& I = march_item<SINGLE>(A, R.meta & 0xffffu);
+                const uint32_t kind = (R.meta >> 16) & 3u, n = (R.meta >> 20) - 0x700u;
+                const MarchItem& I = march_item_uniform<SINGLE>(A, citem);
                 int32_t term = 0;
                 float v = __uint_as_float(kNoHitBits);
                 if (stopped) {
@@ -904,29 +927,45 @@ k_march(MarchArgs A) {
             while (tail - head < 64u && !gen_done) {
                 f3 wd = mk3(0.0f, 0.0f, 0.0f);
                 if (gr >= nrays) {
-                    // next tile: own band first, then the others
-                    uint32_t tile = 0;
-                    bool got = false;
-                    while (misses < 8u) {
-                        uint32_t t = 0;
-                        if (lane == 0) t = atomicAdd(&A.counters[q * kCounterStride], 1u);
-                        t = __builtin_amdgcn_readfirstlane(t);
-                        tile = q * A.band + t;
-                        if (t < A.band && tile < A.ntiles) { got = true; break; }
-                        q = (q + 1u) & 7u;
-                        misses++;
+                    if (!pending) {
+                        // next tile: own band first, then the others
+                        uint32_t tile = 0;
+                        bool got = false;
+                        while (misses < 8u) {
+                            uint32_t t = 0;
+                            if (lane == 0) t = atomicAdd(&A.counters[q * kCounterStride], 1u);
+                            t = __builtin_amdgcn_readfirstlane(t);
+                            tile = q * A.band + t;
+                            if (t < A.band && tile < A.ntiles) { got = true; break; }
+                            q = (q + 1u) & 7u;
+                            misses++;
+                        }
+                        if (!got) { gen_done = true; break; }
+                        gtile = tile;
+                        nitem = __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gtile));
                     }
-                    if (!got) { gen_done = true; break; }
-                    gtile = tile;
-                    gitem = __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gtile));
-                    const MarchItem& I = march_item<SINGLE>(A, gitem);
+                    if (!SINGLE && nitem != citem) {
+                        // another item: switch once nothing of this one is in flight
+                        if (head != tail || __ballot(hasray) != 0) {
+                            pending = true;
+                            break;
+                        }
+                        citem = nitem;
+                        const MarchItem& J = march_item_uniform<SINGLE>(A, citem);
+                        grid = J.grid;
+                        ggx = (uint32_t)J.gx;
+                        ggxy = (uint32_t)J.gxy;
+                    }
+                    pending = false;
+                    gitem = nitem;
+                    const MarchItem& I = march_item_uniform<SINGLE>(A, gitem);
                     uint32_t i = 0;
                     pvalid = tile_lane_pixel(I, gtile - I.tile0, lane, i) && I.T.mat[i] != 0;
                     if (pvalid) pixel_setup(I.T, i, P, wd);
                     gr = 0;
                 }
                 // one batch: ray gr of every valid pixel of the tile
-                const MarchItem& I = march_item<SINGLE>(A, gitem);
+                const MarchItem& I = march_item_uniform<SINGLE>(A, gitem);
                 f3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(1.0f, 1.0f, 1.0f);
                 float maxt = 1.0f;
                 int32_t kind = 0;
@@ -953,7 +992,7 @@ k_march(MarchArgs A) {
                 const uint64_t pm = __ballot(push);
                 if (push)
                     ring_put_ray(ring, (tail + (uint32_t)__popcll(pm & lane_lt)) & (kRing - 1), o, d, cv, kind, slot,
-                                 gitem, (uint32_t)I.gx, (uint32_t)I.gxy);
+                                 (uint32_t)I.gx, (uint32_t)I.gxy);
                 tail += (uint32_t)__popcll(pm);
                 gr++;
             }
@@ -972,7 +1011,6 @@ k_march(MarchArgs A) {
                     R.td2 = __uint_as_float(ring[5][e]);
                     uint32_t xy = ring[6][e], zm = ring[7][e];
                     R.slot = ring[8][e];
-                    uint32_t item = SINGLE ? 0u : ring[9][e];
                     R.x = (int32_t)(xy & 0xffffu);
                     R.y = (int32_t)(xy >> 16) << 2;
                     R.z = (int32_t)(zm & 0xffffu) << 4;
@@ -980,15 +1018,9 @@ k_march(MarchArgs A) {
                     R.sy = (zm & 0x20000u) ? -4 : 4;
                     R.sz = (zm & 0x40000u) ? -16 : 16;
                     uint32_t kind = (zm >> 19) & 3u;
-                    R.meta = (kind << 16) | item;
+                    R.meta = (0x700u << 20) | (kind << 16);
                     R.maxt = kind == RAY_SHADOW ? A.maxt[0] : (kind == RAY_AO ? A.maxt[1] : A.maxt[2]);
-                    if (!SINGLE) {
-                        const MarchItem& I = A.items[item];
-                        grid = I.grid;
-                        ggx = (uint32_t)I.gx;
-                        ggxy = (uint32_t)I.gxy;
-                    }
-                    R.ci = ring[10][e];
+                    R.ci = ring[9][e];
                     R.cw = grid[R.ci];
                     hasray = true;
                     running = true;
@@ -1018,7 +1050,7 @@ k_march(MarchArgs A) {
                     R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
                     R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
                     R.meta += 1u << 20;
-                    if (!(R.meta < (256u << 20) && bt < R.maxt)) running = false;
+                    if (!((int32_t)R.meta >= 0 && bt < R.maxt)) running = false;
                     // the word is fetched only when the ray enters another cell
                     const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
                     if (ci != R.ci) {
@@ -1068,7 +1100,7 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
     float first[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) first[k] = k < A.nrays ? A.res[base + ((size_t)k << 6)] : 0.0f;
-    const MarchItem& I = march_item<SINGLE>(A, __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gt)));
+    const MarchItem& I = march_item_uniform<SINGLE>(A, item_of_tile<SINGLE>(A, gt));
     uint32_t i = 0;
     if (!tile_lane_pixel(I, gt - I.tile0, l, i)) return;
     const TraceArgs& T = I.T;
