@@ -136,6 +136,46 @@ def test_rebuild_instances_match_oracle(api, ctx, oracle):
             idd.prev = idd.cur
 
 
+def test_rebuild_scaled_ragged_and_unaligned_instances_match_oracle(api, ctx, oracle):
+    """A x8 scaled instance (its footprint per block of model voxels is too
+    large for the shared-memory window, so its bits go out one atomic per
+    voxel word), rows that are not whole quads, and a model that starts at an
+    odd address; two frames so the erase matters."""
+    rng = np.random.default_rng(21)
+    S = (40, 36, 44)
+    dims = [(13, 7, 9), (70, 5, 6), (16, 12, 20)]
+    models = [(rng.integers(0, 40, size=d[::-1]) * (rng.random(d[::-1]) < 0.5)).astype(np.uint8) for d in dims]
+    host = np.zeros(S[::-1], np.uint8)
+    occ = api.Volume(ctx, *S, nlevels=1)
+    occ.clear()
+    # the third model sits one byte into its buffer
+    raw = [torch.from_numpy(m.reshape(-1)).cuda() for m in models[:2]]
+    buf = torch.zeros(models[2].size + 1, dtype=torch.uint8, device="cuda")
+    buf[1:] = torch.from_numpy(models[2].reshape(-1)).cuda()
+    ptrs = [raw[0].data_ptr(), raw[1].data_ptr(), buf.data_ptr() + 1]
+    scales = [np.diag([8.0, 8.0, 8.0, 1.0]), _rot(1, 0.6), _rot(0, -0.3) @ np.diag([1.0, 1.2, 0.9, 1.0])]
+    inst_o, inst_d = [], []
+    for k, (mvol, M0) in enumerate(zip(models, scales)):
+        T = np.eye(4)
+        T[:3, 3] = rng.uniform(0.5, 2.0, 3)
+        M = T @ M0
+        Mp = np.eye(4)
+        Mp[:3, 3] = rng.uniform(0.0, 3.0, 3)
+        prev, cur = _colmajor(Mp @ M0), _colmajor(M)
+        piv = rng.uniform(-0.5, 0.5, 3).astype(np.float32)
+        inst_o.append(oracle.make_instance(mvol, prev, cur, piv, 1))
+        inst_d.append(api.make_instance(ptrs[k], mvol.shape[::-1], prev, cur, piv, 1))
+    for frame in range(2):
+        ro = oracle.rebuild(host, inst_o)
+        rd = occ.rebuild(inst_d)
+        assert [tuple(r) for r in ro.tolist()] == [tuple(r) for r in rd], (frame, ro, rd)
+        assert_bitwise(occ.download(), host, f"occupancy frame {frame}")
+        assert host.any()
+        for io, idd in zip(inst_o, inst_d):
+            io.prev = io.cur
+            idd.prev = idd.cur
+
+
 def test_upload_download_roundtrip(api, ctx):
     rng = np.random.default_rng(9)
     S = (20, 12, 28)

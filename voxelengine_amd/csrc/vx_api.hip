@@ -768,29 +768,43 @@ static vx_status vx_volume_rebuild_impl(vx_volume* v, const vx_instance* inst, u
         }
     }
     // March grid over the conservative footprint boxes (fine voxels -> texels):
-    // one refresh per operation, or one over their union for long lists.
+    // one refresh over their union when it is no larger than the boxes together
+    // (an instance's erase and write boxes usually overlap) or the list is
+    // long, else one refresh per operation.
     if (e == hipSuccess) {
-        auto texel_box = [](const PlannedOp& p, int32_t t0[3], int32_t t1[3]) {
+        auto texel_box = [&](const PlannedOp& p, int32_t t0[3], int32_t t1[3]) {
             for (int k = 0; k < 3; ++k) {
                 int64_t lo = std::max<int64_t>(p.lo[k], -8), hi = std::min<int64_t>(p.hi[k], 1ll << 20);
                 t0[k] = (int32_t)(lo >> 1) - 1;
                 t1[k] = (int32_t)(hi >> 1) + 1;
             }
         };
-        if (ops.size() <= 64) {
+        // texels of a box once clipped to the volume (the refresh clips the same way)
+        auto clipped = [&](const int32_t t0[3], const int32_t t1[3]) {
+            const int32_t dims[3] = {v->sx, v->sy, v->sz};
+            double n = 1.0;
+            for (int k = 0; k < 3; ++k)
+                n *= (double)std::max<int64_t>(0, (int64_t)std::min(t1[k], dims[k] - 1) - std::max(t0[k], 0) + 1);
+            return n;
+        };
+        int32_t u0[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, u1[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+        double sum = 0.0;
+        for (const PlannedOp& p : ops) {
+            int32_t t0[3], t1[3];
+            texel_box(p, t0, t1);
+            sum += clipped(t0, t1);
+            for (int k = 0; k < 3; ++k) { u0[k] = std::min(u0[k], t0[k]); u1[k] = std::max(u1[k], t1[k]); }
+        }
+        if (ops.empty()) {
+            // nothing was written
+        } else if (ops.size() > 64 || clipped(u0, u1) <= sum) {
+            e = mgrid_update(v, u0, u1, S(stream));
+        } else {
             for (size_t j = 0; j < ops.size() && e == hipSuccess; ++j) {
                 int32_t t0[3], t1[3];
                 texel_box(ops[j], t0, t1);
                 e = mgrid_update(v, t0, t1, S(stream));
             }
-        } else {
-            int32_t u0[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, u1[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-            for (const PlannedOp& p : ops) {
-                int32_t t0[3], t1[3];
-                texel_box(p, t0, t1);
-                for (int k = 0; k < 3; ++k) { u0[k] = std::min(u0[k], t0[k]); u1[k] = std::max(u1[k], t1[k]); }
-            }
-            e = mgrid_update(v, u0, u1, S(stream));
         }
     }
     if (e == hipSuccess)

@@ -24,6 +24,13 @@ VXD int32_t f2i(float x) {
     if (x < -2147483648.0f) return INT32_MIN;
     return (int32_t)x;
 }
+// The same conversion as one v_cvt_i32_f32, which truncates, saturates to the
+// int32 range and maps NaN to 0 (checked against f2i on special values).
+VXD int32_t f2i_hw(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 VXD float gsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 VXD float gstep(float e, float x) { return x < e ? 0.0f : 1.0f; }
 VXD float gmin(float x, float y) { return y < x ? y : x; }
@@ -181,6 +188,11 @@ VXD size_t brick_index(const OccView& V, int32_t bx, int32_t by, int32_t bz) {
 // Cell word index of fine voxel f (f inside the padded brick grid).
 VXD size_t cell_index(const OccView& V, int32_t fx, int32_t fy, int32_t fz) {
     return brick_index(V, fx >> 3, fy >> 3, fz >> 3) * 8 + sub3(fx >> 2, fy >> 2, fz >> 2);
+}
+// cell_index in 32 bits, for volumes of fewer than 2^31 cell words.
+VXD uint32_t cell_index32(const OccView& V, int32_t fx, int32_t fy, int32_t fz) {
+    return ((((uint32_t)(fz >> 3) * (uint32_t)V.by + (uint32_t)(fy >> 3)) * (uint32_t)V.bx + (uint32_t)(fx >> 3)) << 3) +
+           sub3(fx >> 2, fy >> 2, fz >> 2);
 }
 // Bit of fine voxel f inside its cell word.
 VXD uint32_t cell_bit(int32_t fx, int32_t fy, int32_t fz) {

@@ -169,41 +169,53 @@ __global__ void k_cells_to_bytes(uint8_t* bytes, const uint64_t* cells, OccView 
 // commute (same value, or disjoint footprints). blockIdx.y = operation.
 // A thread takes four consecutive x voxels of a row; voxels that land in
 // the same 32-bit cell word share one atomic. The per-voxel arithmetic is the
-// reference's, voxel by voxel.
+// reference's, voxel by voxel. A thread steps through its quads by a fixed
+// stride, so their row and slice advance by carries instead of divisions.
 __global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, int32_t* aabb, int32_t thr) {
     const FootprintOp op = ops[blockIdx.y];
-    const uint32_t qx = ((uint32_t)op.sx + 3u) >> 2;
-    const uint64_t nq = (uint64_t)qx * (uint32_t)op.sy * (uint32_t)op.sz;
-    const bool narrow = nq <= 0xffffffffull;
+    const uint32_t qx = ((uint32_t)op.sx + 3u) >> 2, sy = (uint32_t)op.sy;
+    const uint64_t nq = (uint64_t)qx * sy * (uint32_t)op.sz;
     // rows of whole quads and a 4-byte aligned model: one dword per quad
     const bool dword = ((uintptr_t)op.model & 3u) == 0 && (op.sx & 3) == 0;
+    const bool narrow_cells = (uint64_t)V.bx * (uint64_t)V.by * (uint64_t)V.bz * 16u < (1ull << 31);
     int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
     uint32_t* words = reinterpret_cast<uint32_t*>(cells);
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // (xq, y, z) of quad q, and the stride as whole rows and slices plus a remainder
+    uint32_t xq, y, z;
+    {
+        const uint64_t r = q / qx;
+        xq = (uint32_t)(q - r * qx);
+        y = (uint32_t)(r % sy);
+        z = (uint32_t)(r / sy);
+    }
+    const uint64_t srow = stride / qx;
+    const uint32_t sxq = (uint32_t)(stride - srow * qx), sy_ = (uint32_t)(srow % sy);
+    const uint32_t sz_ = (uint32_t)(srow / sy);
+    for (; q < nq; q += stride) {
+        const uint32_t cxq = xq, cy = y, cz = z;
+        // the next quad of this thread
+        xq += sxq;
+        const uint32_t carry = xq >= qx ? 1u : 0u;
+        xq -= carry ? qx : 0u;
+        y += sy_ + carry;
+        const uint32_t carry2 = y >= sy ? 1u : 0u;
+        y -= carry2 ? sy : 0u;
+        z += sz_ + carry2;
+
         uint32_t m4 = 0;
+        const uint32_t nvalid = min(4u, (uint32_t)op.sx - 4u * cxq);
         if (dword) {
             m4 = reinterpret_cast<const uint32_t*>(op.model)[q];
             if ((int32_t)(m4 & 255u) < thr && (int32_t)((m4 >> 8) & 255u) < thr &&
                 (int32_t)((m4 >> 16) & 255u) < thr && (int32_t)(m4 >> 24) < thr)
                 continue;
-        }
-        uint32_t xq, y, z;
-        if (narrow) {
-            const uint32_t q32 = (uint32_t)q, r = q32 / qx;
-            xq = q32 - r * qx;
-            y = r % (uint32_t)op.sy;
-            z = r / (uint32_t)op.sy;
         } else {
-            const uint64_t r = q / qx;
-            xq = (uint32_t)(q - r * qx);
-            y = (uint32_t)(r % (uint32_t)op.sy);
-            z = (uint32_t)(r / (uint32_t)op.sy);
-        }
-        const uint32_t nvalid = min(4u, (uint32_t)op.sx - 4u * xq);
-        if (!dword) {
-            const uint8_t* p = op.model + (((size_t)z * (uint32_t)op.sy + y) * (uint32_t)op.sx + 4u * xq);
+            const uint8_t* p = op.model + (((size_t)cz * sy + cy) * (uint32_t)op.sx + 4u * cxq);
             for (uint32_t j = 0; j < nvalid; ++j) m4 |= (uint32_t)p[j] << (8 * j);
         }
+        const float fy = (float)cy, fz = (float)cz;
         size_t wa[4];
         uint32_t wm[4];
         bool has[4];
@@ -213,19 +225,22 @@ __global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, 
             wa[j] = 0;
             wm[j] = 0;
             if ((uint32_t)j >= nvalid || (int32_t)((m4 >> (8 * j)) & 255u) < thr) continue;
-            const int32_t x = (int32_t)(4 * xq + j);
+            const float fx = (float)(int32_t)(4 * cxq + j);
             int32_t f[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                float wp = ((op.o[k] + (float)x * op.dx[k]) + (float)y * op.dy[k]) + (float)z * op.dz[k];
-                f[k] = f2i(wp * 10.0f);
+                float wp = ((op.o[k] + fx * op.dx[k]) + fy * op.dy[k]) + fz * op.dz[k];
+                f[k] = f2i_hw(wp * 10.0f);
                 mn[k] = min(mn[k], f[k]);
                 mx[k] = max(mx[k], f[k]);
             }
             // SetVolumeAt (ShadowVoxSystem.cpp:82-94): out-of-range writes are dropped.
-            if (f[0] < 0 || f[1] < 0 || f[2] < 0 || f[0] >= 2 * V.sx || f[1] >= 2 * V.sy || f[2] >= 2 * V.sz) continue;
+            if ((uint32_t)f[0] >= 2u * (uint32_t)V.sx || (uint32_t)f[1] >= 2u * (uint32_t)V.sy ||
+                (uint32_t)f[2] >= 2u * (uint32_t)V.sz)
+                continue;
             const uint32_t bit = cell_bit(f[0], f[1], f[2]);
-            wa[j] = cell_index(V, f[0], f[1], f[2]) * 2 + (bit >> 5);
+            wa[j] = (narrow_cells ? (size_t)cell_index32(V, f[0], f[1], f[2]) : cell_index(V, f[0], f[1], f[2])) * 2 +
+                    (bit >> 5);
             wm[j] = 1u << (bit & 31u);
             has[j] = true;
         }
@@ -242,8 +257,11 @@ __global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, 
             else atomicAnd(words + wa[j], ~wm[j]);
         }
     }
-    // AABB of the footprint (ShadowVoxSystem.cpp:148-149): wave reduction, one
-    // atomic per wave into the instance's slot.
+    // AABB of the footprint (ShadowVoxSystem.cpp:148-149): wave reduction, then
+    // at most one atomic per bound and wave. Every wave targets the same six
+    // words, whose atomics serialize, so a bound that the slot already holds is
+    // skipped; the slot is read without an atomic, and a stale value is only
+    // ever larger for a min (smaller for a max), which costs an atomic, never a bound.
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         int32_t a = mn[k], b = mx[k];
@@ -252,8 +270,10 @@ __global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, 
             b = max(b, __shfl_xor(b, off));
         }
         if ((threadIdx.x & 63) == 0) {
-            if (a != INT32_MAX) atomicMin(&aabb[op.inst * 6 + k], a);
-            if (b != INT32_MIN) atomicMax(&aabb[op.inst * 6 + 3 + k], b);
+            int32_t* lo = &aabb[op.inst * 6 + k];
+            int32_t* hi = &aabb[op.inst * 6 + 3 + k];
+            if (a != INT32_MAX && a < __builtin_nontemporal_load(lo)) atomicMin(lo, a);
+            if (b != INT32_MIN && b > __builtin_nontemporal_load(hi)) atomicMax(hi, b);
         }
     }
 }
