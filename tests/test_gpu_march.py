@@ -135,3 +135,54 @@ def test_march_profile_counts_launches(api, ctx):
     ms, n = ctx.profile_read()
     ctx.profile(False)
     assert n == 3 and ms > 0.0
+
+
+def test_no_rays_one_ray_class_and_tiny_views(api, oracle):
+    """Zero rays per pixel (no march: k_resolve writes lit / 1 / 1), one ray
+    class at a time, and 1x1, sub-tile and ragged views, starting on a fresh
+    context whose trace buffers do not exist yet; then the same through a
+    batch."""
+    from voxelengine_amd import configs
+    ctx = api.Context(0)
+    model = _model(api, ctx, (40, 32, 36), seed=5)
+    vol = model.download(0)
+    dims = (20, 16, 18)
+    occ = api.Volume(ctx, *dims, nlevels=1)
+    occ.clear()
+    occ.rebuild([api.make_instance(model.mip_ptr(0), model.dims, IDENTITY, IDENTITY)])
+    occ.build_levels()
+    occ_h = np.zeros(dims[::-1], np.uint8)
+    oracle.rebuild(occ_h, [oracle.make_instance(vol, IDENTITY, IDENTITY)])
+    for counts in [(0, 0, 0), (0, 3, 0), (1, 0, 0), (0, 0, 1)]:
+        params = configs.trace_params(n_shadow=counts[0], n_ao=counts[1], n_spec=counts[2], ao_maxt=20.0,
+                                      march="dda")
+        for w, h in [(1, 1), (7, 3), (16, 9)]:
+            got, ref = _trace_both(api, ctx, oracle, occ, occ_h, model, 1, params, w, h)
+            _assert_same(got, ref, f"rays {counts} view {w}x{h}")
+            if counts == (0, 0, 0):
+                # hit pixels read lit / 1 / 1, sky pixels 0 / 0 / 0
+                lit = got[0] == 1
+                assert np.array_equal(got[1] == 1.0, lit) and np.array_equal(got[2] == 1.0, lit)
+                assert not got[3].any()
+    # a batch without rays, and a batch of tiny views
+    from voxelengine_amd import views
+    pal = torch.from_numpy(configs.make_palette()).cuda()
+    for counts in [(0, 0, 0), (1, 2, 1)]:
+        params = configs.trace_params(n_shadow=counts[0], n_ao=counts[1], n_spec=counts[2], ao_maxt=20.0,
+                                      march="dda")
+        items, plain = [], []
+        for k, (w, h) in enumerate([(1, 1), (7, 3), (16, 9), (3, 17)]):
+            vd = views.scene_view(model.dims, w, h, frame=k, variant=k)
+            g = api.GBuffer(w, h)
+            api.primary(ctx, model, vd, g)
+            p = api.Visibility(w, h, fetches=True)
+            api.trace(ctx, occ, vd, g, pal, params, p)
+            items.append((occ, vd, g, api.Visibility(w, h, fetches=True)))
+            plain.append(p)
+        batch = api.Batch(ctx, items, pal, params)
+        batch.launch()
+        torch.cuda.synchronize()
+        for k, ((_, _, _, o), p) in enumerate(zip(items, plain)):
+            for f in ("shadow", "ao", "spec", "fetches"):
+                assert_bitwise(getattr(o, f).cpu().numpy(), getattr(p, f).cpu().numpy(), f"batch {counts} item {k} {f}")
+        batch.close()

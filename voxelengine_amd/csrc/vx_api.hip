@@ -1152,6 +1152,7 @@ static vx_status persistent_grid(vx_ctx* ctx, const void* kernel, int block, siz
 // Grows the ray-parallel scratch to `slots` ray slots (and fetch counts when
 // asked); buffers in use by queued work are released after a device sync.
 static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
+    if (!m.counters) VX_HIP(hipMalloc(&m.counters, kCounterWords * sizeof(uint32_t)));
     const bool grow = m.slots < slots, fgrow = fetches && m.fslots < slots;
     if (!grow && !fgrow) return VX_OK;
     VX_HIP(hipDeviceSynchronize());
@@ -1172,7 +1173,6 @@ static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
         VX_HIP(hipMalloc(&m.fres, slots * sizeof(uint16_t)));
         m.fslots = slots;
     }
-    if (!m.counters) VX_HIP(hipMalloc(&m.counters, kCounterWords * sizeof(uint32_t)));
     return VX_OK;
 }
 

@@ -903,7 +903,7 @@ k_march(MarchArgs A) {
 
     // generation cursor (wave-uniform)
     uint32_t q = blockIdx.x & 7u, misses = 0;
-    bool gen_done = false;
+    bool gen_done = nrays <= 0;   // no rays per pixel: k_resolve writes the defaults
     uint32_t gitem = 0, gtile = 0;
     int32_t gr = nrays;
     uint32_t head = 0, tail = 0;
