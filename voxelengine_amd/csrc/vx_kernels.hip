@@ -1345,8 +1345,13 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
 // 1, so the voxel step ivec3(select * stepSign) is taken as an integer select;
 // t_max, best_t and the normal keep the reference's float products (an
 // infinite t_delta times a 0 select is NaN there too).
-VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t frame, f3 o3, f3 d3, f3 cam,
-                             float uvx, float uvy, f3& hit, f3& nrm, uint32_t& material) {
+// WIDTH: 1 when every level's voxel index fits 32 bits (mip 0 holds fewer than
+// 2^32 voxels), 2 when it does not, 0 to decide at each step. The single-model
+// pass takes the fixed forms; the instance pass keeps the test in the loop,
+// which leaves it fewer registers and more waves.
+template <int WIDTH>
+VXD int32_t intersect_volume_t(const ModelMips& A, const float res[2], int32_t frame, f3 o3, f3 d3, f3 cam,
+                               float uvx, float uvy, f3& hit, f3& nrm, uint32_t& material) {
     float o[3] = {o3.x, o3.y, o3.z}, d[3] = {d3.x, d3.y, d3.z}, c[3] = {cam.x, cam.y, cam.z};
     float s[3], td[3];
     int32_t is[3];
@@ -1356,8 +1361,8 @@ VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t fra
         td[k] = 1.0f / (d[k] * s[k]);
         is[k] = (int32_t)s[k];
     }
-    // voxel indices fit 32 bits unless mip 0 holds 2^32 voxels or more
-    const bool narrow = (uint64_t)(uint32_t)A.dims[0] * (uint32_t)A.dims[1] * (uint32_t)A.dims[2] < (1ull << 32);
+    const bool narrow =
+        WIDTH == 1 || (WIDTH == 0 && (uint64_t)(uint32_t)A.dims[0] * (uint32_t)A.dims[1] * (uint32_t)A.dims[2] < (1ull << 32));
     int32_t mip = 2;
     int32_t i = 0;
     do {
@@ -1384,16 +1389,22 @@ VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t fra
             cv[0] += b0 ? is[0] : 0;
             cv[1] += b1 ? is[1] : 0;
             cv[2] += b2 ? is[2] : 0;
-            if (cv[0] < -1 || cv[0] > up[0] || cv[1] < -1 || cv[1] > up[1] || cv[2] < -1 || cv[2] > up[2])
-                return 0;
-            uint32_t vox = 0u;
-            if ((uint32_t)cv[0] < mx && (uint32_t)cv[1] < my && (uint32_t)cv[2] < mz) {
-                if (narrow)
-                    vox = base[(uint32_t)cv[0] + (uint32_t)cv[1] * mx + (uint32_t)cv[2] * (mx * my)];
-                else
-                    vox = base[(size_t)cv[0] + (size_t)cv[1] * mx + (size_t)cv[2] * mx * my];
+            const bool oob =
+                cv[0] < -1 || cv[0] > up[0] || cv[1] < -1 || cv[1] > up[1] || cv[2] < -1 || cv[2] > up[2];
+            // the texel is read unconditionally (byte 0 of the level for a cell
+            // outside it, then taken as empty), so the step has one branch
+            const bool inside = (uint32_t)cv[0] < mx && (uint32_t)cv[1] < my && (uint32_t)cv[2] < mz;
+            uint32_t vox;
+            if (narrow) {
+                const uint32_t idx = (uint32_t)cv[0] + (uint32_t)cv[1] * mx + (uint32_t)cv[2] * (mx * my);
+                vox = base[inside ? idx : 0u];
+            } else {
+                const size_t idx = (size_t)(uint32_t)cv[0] + (size_t)(uint32_t)cv[1] * mx + (size_t)(uint32_t)cv[2] * mx * my;
+                vox = base[inside ? idx : (size_t)0];
             }
-            if (vox != 0u) {
+            vox = inside ? vox : 0u;
+            if (oob || vox != 0u) {
+                if (oob) return 0;
                 float best_t = (tm[0] * s0 + tm[1] * s1) + tm[2] * s2;
                 bool accept = (mip == 0);
                 if (!accept) {
@@ -1427,6 +1438,13 @@ VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t fra
         for (int k = 0; k < 3; ++k) o[k] = o[k] * mip_size;
     } while (++i < 4);
     return 0;
+}
+
+VXD int32_t intersect_volume(const ModelMips& A, const float res[2], int32_t frame, f3 o3, f3 d3, f3 cam,
+                             float uvx, float uvy, f3& hit, f3& nrm, uint32_t& material) {
+    if ((uint64_t)(uint32_t)A.dims[0] * (uint32_t)A.dims[1] * (uint32_t)A.dims[2] < (1ull << 32))
+        return intersect_volume_t<1>(A, res, frame, o3, d3, cam, uvx, uvy, hit, nrm, material);
+    return intersect_volume_t<2>(A, res, frame, o3, d3, cam, uvx, uvy, hit, nrm, material);
 }
 
 // Pinhole ray of pixel (px, py) from the inverse projection / view
@@ -1574,7 +1592,7 @@ __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A) {
         const f3 start = clip_to_aabb(lc, dl, I.m.dims);
         f3 hit, nrm;
         uint32_t mat = 0;
-        if (!intersect_volume(I.m, vw.res, vw.frame, start, dl, lc, ndc_x, ndc_y, hit, nrm, mat)) continue;
+        if (!intersect_volume_t<0>(I.m, vw.res, vw.frame, start, dl, lc, ndc_x, ndc_y, hit, nrm, mat)) continue;
         float h4[4] = {hit.x * 0.1f, hit.y * 0.1f, hit.z * 0.1f, 1.0f}, wh[4], cur[4];
         mat_vec(I.w, h4, wh);
         mat_vec(A.pv, wh, cur);
