@@ -927,7 +927,7 @@ k_march(MarchArgs A) {
         if (runm == 0 || (uint32_t)__popcll(~runm) >= (uint32_t)A.refill) {
             // ---- finish ended rays
             if (hasray && !running) {
-                const uint32_t kind = (R.meta >> 16) & 3u, n = (R.meta >> 20) - 0x700u;
+                const uint32_t kind = (R.meta >> 16) & 3u, n = (R.meta >> 20) - 0x700u - (stopped ? 1u : 0u);
                 const MarchItem& I = march_item_uniform<SINGLE>(A, citem);
                 int32_t term = 0;
                 float v = __uint_as_float(kNoHitBits);
@@ -1054,29 +1054,33 @@ k_march(MarchArgs A) {
 #pragma unroll
         for (int u = 0; u < STEPS; ++u) {
             if (running) {
-                // grid words are bit-reversed: voxel bit b sits at bit 63 - b
-                if ((int64_t)(R.cw << (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) < 0) {
-                    running = false;
-                    stopped = true;
-                } else {
-                    float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
-                    bool s0 = R.tm0 == tmin, s1 = R.tm1 == tmin, s2 = R.tm2 == tmin;
-                    float bt = tmin;
-                    if ((s0 && s1) || (s2 && (s0 || s1))) bt = (s0 && s1 && s2) ? (tmin + tmin) + tmin : tmin + tmin;
-                    R.x = s0 ? R.x + R.sx : R.x;
-                    R.y = s1 ? R.y + R.sy : R.y;
-                    R.z = s2 ? R.z + R.sz : R.z;
-                    R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
-                    R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
-                    R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
-                    R.meta += 1u << 20;
-                    if (!((int32_t)R.meta >= 0 && bt < R.maxt)) running = false;
-                    // the word is fetched only when the ray enters another cell
-                    const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
-                    if (ci != R.ci) {
-                        R.ci = ci;
-                        R.cw = grid[ci];
-                    }
+                // grid words are bit-reversed: voxel bit b sits at bit 63 - b. An
+                // occupied cell stops the ray where it is: no axis steps, and its
+                // step count (counted here) loses that step when it finishes.
+                const bool occ =
+                    (int64_t)(R.cw << (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) < 0;
+                const float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
+                const bool s0 = R.tm0 == tmin && !occ, s1 = R.tm1 == tmin && !occ, s2 = R.tm2 == tmin && !occ;
+                float bt = tmin;
+                // two or three axes step together (rare): the median equals the minimum
+                if (__builtin_expect(__builtin_amdgcn_fmed3f(R.tm0, R.tm1, R.tm2) == tmin, 0)) {
+                    const float t2 = tmin + tmin;
+                    bt = (R.tm0 == tmin && R.tm1 == tmin && R.tm2 == tmin) ? t2 + tmin : t2;
+                }
+                R.x = s0 ? R.x + R.sx : R.x;
+                R.y = s1 ? R.y + R.sy : R.y;
+                R.z = s2 ? R.z + R.sz : R.z;
+                R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
+                R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
+                R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
+                R.meta += 1u << 20;
+                running = !occ && (int32_t)R.meta >= 0 && bt < R.maxt;
+                stopped = occ;
+                // the word is fetched only when the ray enters another cell
+                const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
+                if (ci != R.ci) {
+                    R.ci = ci;
+                    R.cw = grid[ci];
                 }
             }
         }
