@@ -894,6 +894,7 @@ VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gx
 template <int BLOCK, int STEPS, bool SINGLE>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)))
 k_march(MarchArgs A) {
+    static_assert(256 % STEPS == 0, "the step limit is tested at block ends");
     constexpr int WAVES = BLOCK / 64;
     __shared__ uint32_t ring_s[WAVES][kRingFields][kRing];
     const uint32_t lane = threadIdx.x & 63u;
@@ -1074,7 +1075,11 @@ k_march(MarchArgs A) {
                 R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
                 R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
                 R.meta += 1u << 20;
-                running = !occ && (int32_t)R.meta >= 0 && bt < R.maxt;
+                // A ray enters the march between blocks of STEPS steps with a
+                // count of 0 and steps once per iteration while it runs, so its
+                // count reaches 256 (a multiple of STEPS) only at a block's last
+                // step: the limit is tested there alone.
+                running = !occ && (u != STEPS - 1 || (int32_t)R.meta >= 0) && bt < R.maxt;
                 stopped = occ;
                 // the word is fetched only when the ray enters another cell
                 const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
