@@ -8,7 +8,10 @@ the view; inputs (volume, levels, G-buffer) are resident in HBM before timing.
 
 Prints one JSON line. Extra fields: the reference-marcher variant of the same
 frame, the backward (shading adjoint) rate, rebuild and primary-hit times,
-the HBM roofline of the trace kernel and the CPU oracle baseline.
+the point/spot light pass (64 + 64 lights) over the same frame, config B
+(batched against plain launches), the prefabs (load, rebuild, one frame with
+geometry pass, trace and temporal accumulation), config L, the HBM roofline
+of the trace kernel and the CPU oracle baseline.
 """
 from __future__ import annotations
 
@@ -240,6 +243,60 @@ def _prefab_frame(ctx, P, models, vol, reps=10):
             "taa_mpx_per_s": round(W * H / (taa_ms * 1e-3) / 1e6, 1)}
 
 
+def bench_lights(ctx, occ, vd, g, pal_d, dims, reps=10):
+    """The point/spot light pass over the headline frame with 64 point and 64
+    spot lights, the reference's limit per type (LightPointPipeline.h:15,
+    LightSpotPipeline.h), marched with the reference's shipped marchers
+    (Sparse for point, SuperSparse for spot). A shadow ray is marched for every
+    (hit pixel, light in range) pair; that count comes from the same test on
+    the host in float64, and the fetch count from one counting launch."""
+    import torch
+    from voxelengine_amd import api, configs
+    rng = np.random.default_rng(configs.SEED + 7)
+    ext = np.array(dims, np.float64) * 0.1                   # world extent of the scene
+    L = np.zeros(128, api.LIGHT_DTYPE)
+    for k in range(L.size):
+        L[k]["type"] = 0 if k < 64 else 1
+        L[k]["position"] = rng.uniform([0.0, 0.3, 0.0], [1.0, 0.7, 1.0]) * ext
+        L[k]["range"] = rng.uniform(0.1, 0.3) * ext.max()
+        L[k]["color"] = rng.uniform(0.2, 4.0, 3)
+        L[k]["attenuation"] = rng.uniform(0.5, 2.0)
+        d = rng.normal(size=3)
+        d[1] = -abs(d[1])
+        L[k]["direction"] = d / np.linalg.norm(d)
+        L[k]["angle"] = rng.uniform(0.2, 0.9)
+        L[k]["angle_attenuation"] = rng.uniform(0.5, 3.0)
+    lights = api.lights_to_device(L)
+    npx = vd["width"] * vd["height"]
+    hit = (g.material > 0)
+    wpos = g.pos[hit].double() * 0.1
+    lp = torch.from_numpy(np.ascontiguousarray(L["position"], np.float64)).cuda()
+    lr = torch.from_numpy(np.ascontiguousarray(L["range"], np.float64)).cuda()
+    rays = 0
+    for k0 in range(0, L.size, 16):
+        dist = torch.cdist(wpos, lp[k0:k0 + 16])
+        rays += int((dist <= lr[k0:k0 + 16]).sum().item())
+    rgb = torch.zeros((npx, 3), dtype=torch.float32, device="cuda")
+    fetches = torch.zeros(npx, dtype=torch.int32, device="cuda")
+    api.light_pass(ctx, occ, vd, g, pal_d, lights, L.size, rgb, seed=3, frame=1, fetches=fetches)
+    torch.cuda.synchronize()
+    s_ref = int(fetches.to(torch.int64).sum().item())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        api.light_pass(ctx, occ, vd, g, pal_d, lights, L.size, rgb, seed=3, frame=1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # algorithmic bytes: one byte per reference fetch, plus per hit pixel the
+    # G-buffer read (25 B) and the rgb read and write (24 B)
+    alg = s_ref + 49 * int(hit.sum().item())
+    return {"point_lights": 64, "spot_lights": 64, "marchers": "sparse (point) / supersparse (spot)",
+            "ms": round(ms, 4), "shadow_rays": rays, "grays_per_s": round(rays / (ms * 1e-3) / 1e9, 3),
+            "ref_fetches": s_ref, "achieved_gbps": round(alg / (ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm": round(alg / (ms * 1e-3) / 8.0e12, 4)}
+
+
 def bench_prefabs(ctx, reps=5):
     """The reference's FarmHouse and ModernHouse prefabs (files rebuilt from
     tests/golden/prefabs.npz) through the loaders into the 524x188x524 volume:
@@ -440,6 +497,8 @@ def main():
                            "px_per_s": round(hit_px / (bwd_ms * 1e-3), 1)}
 
     if not args.no_extras:
+        log("light pass ...")
+        out["lights"] = bench_lights(ctx, occ, vd, g, pal_d, cfg["dims"])
         out["batched"] = bench_batched(ctx, pal_d)
         log("prefabs ...")
         out["prefabs"] = bench_prefabs(ctx)

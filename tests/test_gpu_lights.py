@@ -51,3 +51,35 @@ def test_light_pass_matches_oracle(ctx, oracle, marchers):
     assert_bitwise(fetches.cpu().numpy().astype(np.uint32), ref_f, "fetches")
     lit = (ref != base).any(axis=1)
     assert lit.mean() > 0.2 and (~lit & (g.material.cpu().numpy() > 0)).any()
+
+
+def test_many_small_lights_match_oracle(ctx, oracle):
+    """150 lights with small ranges, so each 16x16 tile keeps only a few of them
+    (the per-tile cull), plus ranges that are infinite, negative and zero:
+    rgb and fetch counts bit for bit against the oracle, which tests every
+    light at every pixel. (A NaN range, kept by the cull, makes the pixel's
+    sum NaN, whose bits the two sides need not share; it is left out.)"""
+    from voxelengine_amd import api, configs, views
+    model, occ, vol = device_scene(api, ctx, 64, seed=32, fill=0.3, nboxes=20)
+    occ_h = oracle_occ(oracle, vol)
+    w, h = 120, 88
+    vd = views.scene_view(model.dims, w, h, variant=1)
+    g = api.GBuffer(w, h)
+    api.primary(ctx, model, vd, g)
+    pal = configs.make_palette()
+    L = _lights(model.dims, n_point=80, n_spot=70, seed=9)
+    rng = np.random.default_rng(10)
+    L["range"] = rng.uniform(0.05, 0.25, L.size) * np.float32(model.dims[0] * 0.1)
+    L["range"][[40, 90, 120]] = [np.inf, -1.0, 0.0]
+    base = np.random.default_rng(11).random((w * h, 3)).astype(np.float32)
+    rgb = torch.from_numpy(base.copy()).cuda()
+    fetches = torch.zeros(w * h, dtype=torch.int32, device="cuda")
+    api.light_pass(ctx, occ, vd, g, torch.from_numpy(pal).cuda(), api.lights_to_device(L), L.size, rgb,
+                   seed=5, frame=2, fetches=fetches)
+    ref, ref_f = oracle.light_pass(occ_h, vd, L, oracle.light_params(1, 2, 5, 2), g.pos.cpu().numpy(),
+                                   g.normal.cpu().numpy(), g.material.cpu().numpy(), pal, rgb=base)
+    got = rgb.cpu().numpy()
+    assert_bitwise(got, ref, "rgb")
+    assert_bitwise(fetches.cpu().numpy().astype(np.uint32), ref_f, "fetches")
+    lit = (ref != base).any(axis=1)
+    assert lit.mean() > 0.02

@@ -1264,22 +1264,94 @@ __global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32
 // ===========================================================================
 // Point and spot lights (LightPoint.frag:85-152, LightSpot.frag:73-140): one
 // thread per pixel, the lights' draws as a loop in draw order, world space.
+//
+// A block (a 16x16 tile) first bounds its hit pixels' world positions and
+// keeps, in draw order, the lights whose sphere (range plus a wide margin)
+// reaches that box; every pixel still applies the reference's range test, so
+// a light dropped here is one the test would skip at every pixel of the tile.
+// A tile with a non-finite position keeps every light.
 // ===========================================================================
+constexpr uint32_t kMaxLights = 4096;
+
 __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
+    __shared__ uint16_t list_s[kMaxLights];
+    __shared__ float box_s[4][6];
+    __shared__ uint32_t wcount_s[4];
+    __shared__ uint32_t bad_s;
+    const uint32_t logical = (blockIdx.x % 8u) * A.tiles.per_xcd + blockIdx.x / 8u;
+    if (logical >= A.tiles.ntiles) return;   // block-uniform
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
     int32_t px, py;
-    if (!tile_pixel(A.tiles, px, py)) return;
-    const size_t i = (size_t)py * A.view.width + px;
-    const uint32_t m = A.mat[i];
-    if (m == 0) {
+    const bool inview = tile_pixel(A.tiles, px, py);
+    const size_t i = inview ? (size_t)py * A.view.width + px : 0;
+    const uint32_t m = inview ? A.mat[i] : 0u;
+    const bool valid = m != 0;
+    f3 pp = mk3(0.0f, 0.0f, 0.0f);
+    if (valid) pp = load3(A.pos + 3 * i);
+    const f3 wpos = mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f);
+    // the tile's box of world positions
+    if (t == 0) bad_s = 0;
+    const bool finite = isfinite(wpos.x) && isfinite(wpos.y) && isfinite(wpos.z);
+    float lo[3] = {valid ? wpos.x : INFINITY, valid ? wpos.y : INFINITY, valid ? wpos.z : INFINITY};
+    float hi[3] = {valid ? wpos.x : -INFINITY, valid ? wpos.y : -INFINITY, valid ? wpos.z : -INFINITY};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off));
+        }
+    }
+    const bool wave_bad = __ballot(valid && !finite) != 0;
+    __syncthreads();   // bad_s cleared
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { box_s[wave][k] = lo[k]; box_s[wave][3 + k] = hi[k]; }
+        if (wave_bad) bad_s = 1;
+    }
+    __syncthreads();
+    float blo[3], bhi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        blo[k] = fminf(fminf(box_s[0][k], box_s[1][k]), fminf(box_s[2][k], box_s[3][k]));
+        bhi[k] = fmaxf(fmaxf(box_s[0][3 + k], box_s[1][3 + k]), fmaxf(box_s[2][3 + k], box_s[3][3 + k]));
+    }
+    const bool keep_all = bad_s != 0;
+    // the lights that can reach the box, in draw order: 256 at a time, each
+    // wave's survivors after those of the waves before it
+    uint32_t n = 0;
+    for (uint32_t c0 = 0; c0 < A.nlights; c0 += 256) {
+        const uint32_t l = c0 + t;
+        bool keep = false;
+        if (l < A.nlights) {
+            const vx_light& L = A.lights[l];
+            float d2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float q = fmaxf(fmaxf(blo[k] - L.position[k], L.position[k] - bhi[k]), 0.0f);
+                d2 = d2 + q * q;
+            }
+            const float r = L.range * 1.001f + 0.01f;
+            keep = keep_all || !(d2 > r * r);   // NaN range or box: kept
+        }
+        const uint64_t km = __ballot(keep);
+        if (lane == 0) wcount_s[wave] = (uint32_t)__popcll(km);
+        __syncthreads();
+        uint32_t base = n;
+        for (uint32_t w = 0; w < wave; ++w) base += wcount_s[w];
+        if (keep) list_s[base + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))] = (uint16_t)l;
+        n += (wcount_s[0] + wcount_s[1]) + (wcount_s[2] + wcount_s[3]);
+        __syncthreads();   // counts read, list written
+    }
+    if (!inview) return;
+    if (!valid) {
         if (A.fetches) A.fetches[i] = 0;
         return;
     }
-    const f3 pp = load3(A.pos + 3 * i), N = load3(A.nrm + 3 * i);
+    const f3 N = load3(A.nrm + 3 * i);
     const uint32_t pk = pixkey(A.params.seed, A.params.frame, (uint32_t)i);
     float r0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) r0[k] = rand01(pk, (uint32_t)k);
-    const f3 wpos = mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f);
     const f3 cam = mk3(A.view.cam_pos[0] * 10.0f, A.view.cam_pos[1] * 10.0f, A.view.cam_pos[2] * 10.0f);
     const f3 Vd = normalize3(mk3(cam.x - pp.x, cam.y - pp.y, cam.z - pp.z));
     const float* e = A.palette + 6 * m;
@@ -1296,8 +1368,8 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
     rv.z = rv.z * gsign(r0[2] - 0.5f);
     float acc[3] = {A.rgb[3 * i + 0], A.rgb[3 * i + 1], A.rgb[3 * i + 2]};
     uint32_t nf = 0;
-    for (uint32_t l = 0; l < A.nlights; ++l) {
-        const vx_light& L = A.lights[l];
+    for (uint32_t k = 0; k < n; ++k) {
+        const vx_light& L = A.lights[__builtin_amdgcn_readfirstlane(list_s[k])];
         const f3 ld = mk3(L.position[0] - wpos.x, L.position[1] - wpos.y, L.position[2] - wpos.z);
         const float dist = sqrtf(dot3(ld, ld));
         if (dist > L.range) continue;
