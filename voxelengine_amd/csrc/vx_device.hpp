@@ -207,11 +207,15 @@ VXD uint32_t spread_byte(uint32_t b) {
     return (b & 3u) | ((b & 0xcu) << 2) | ((b & 0x30u) << 12) | ((b & 0xc0u) << 14);
 }
 // Reference byte of texel t (0 when outside [0, s)), texelFetch semantics.
+// Texels outside the volume read 0; the word is read unconditionally (cell
+// word 0 for those), so a march step carries no branch for it.
 VXD uint32_t texel_byte(const OccView& V, int32_t tx, int32_t ty, int32_t tz) {
-    if ((uint32_t)tx >= (uint32_t)V.sx || (uint32_t)ty >= (uint32_t)V.sy || (uint32_t)tz >= (uint32_t)V.sz) return 0u;
-    uint64_t w = V.cells[brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1)];
-    uint32_t x = (uint32_t)(w >> texel_base(tx, ty, tz));
-    return (x & 3u) | ((x >> 2) & 0xcu) | ((x >> 12) & 0x30u) | ((x >> 14) & 0xc0u);
+    const bool in = (uint32_t)tx < (uint32_t)V.sx && (uint32_t)ty < (uint32_t)V.sy && (uint32_t)tz < (uint32_t)V.sz;
+    const size_t ci = in ? brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1) : 0;
+    const uint64_t w = V.cells[ci];
+    const uint32_t x = (uint32_t)(w >> texel_base(tx, ty, tz));
+    const uint32_t b = (x & 3u) | ((x >> 2) & 0xcu) | ((x >> 12) & 0x30u) | ((x >> 14) & 0xc0u);
+    return in ? b : 0u;
 }
 VXD bool level_bit(const OccView& V, int32_t lvl, int32_t cx, int32_t cy, int32_t cz) {
     size_t wi = ((size_t)(cz >> 2) * V.wy[lvl] + (cy >> 2)) * V.wx[lvl] + (cx >> 2);
