@@ -18,15 +18,10 @@ namespace vx {
 
 #define VXD __device__ __forceinline__
 
+// ivec3(x): truncation toward zero, saturating to the int32 range, NaN -> 0.
+// That is v_cvt_i32_f32 exactly (checked on NaN, infinities, values at and
+// past the range and denormals); a C++ cast would leave those cases undefined.
 VXD int32_t f2i(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return INT32_MAX;
-    if (x < -2147483648.0f) return INT32_MIN;
-    return (int32_t)x;
-}
-// The same conversion as one v_cvt_i32_f32, which truncates, saturates to the
-// int32 range and maps NaN to 0 (checked against f2i on special values).
-VXD int32_t f2i_hw(float x) {
     int32_t r;
     asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
     return r;

@@ -230,7 +230,7 @@ __global__ void k_footprint(uint64_t* cells, OccView V, const FootprintOp* ops, 
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 float wp = ((op.o[k] + fx * op.dx[k]) + fy * op.dy[k]) + fz * op.dz[k];
-                f[k] = f2i_hw(wp * 10.0f);
+                f[k] = f2i(wp * 10.0f);
                 mn[k] = min(mn[k], f[k]);
                 mx[k] = max(mx[k], f[k]);
             }
@@ -424,7 +424,7 @@ VXD float sparse_march(const OccView& V, f3 o, f3 d, float dist, float step_fact
     R.hit = 0; R.term = 0;
     while (dd < 16.0f) {
         it++;
-        uint32_t vol = texel_byte(V, f2i_hw(pos.x / 2.0f), f2i_hw(pos.y / 2.0f), f2i_hw(pos.z / 2.0f));
+        uint32_t vol = texel_byte(V, f2i(pos.x / 2.0f), f2i(pos.y / 2.0f), f2i(pos.z / 2.0f));
         uint32_t bit = 0u;
         bit += gmod(pos.x, 0.5f) > 0.25f ? 1u : 0u;
         bit += gmod(pos.y, 0.5f) > 0.25f ? 2u : 0u;
@@ -442,7 +442,7 @@ VXD float sparse_march(const OccView& V, f3 o, f3 d, float dist, float step_fact
     while (dd < lim) {
         it++;
         // getVolumeAt(ivec3(pos), 1): byte of texel ivec3(pos)/2 is non-zero.
-        int32_t ix = f2i_hw(pos.x), iy = f2i_hw(pos.y), iz = f2i_hw(pos.z);
+        int32_t ix = f2i(pos.x), iy = f2i(pos.y), iz = f2i(pos.z);
         if (texel_byte(V, ix / 2, iy / 2, iz / 2) != 0u) {
             R.hit = 1; R.term = 1; R.steps = R.fetches = it; R.t = dd;
             return dd;
