@@ -186,3 +186,40 @@ def test_no_rays_one_ray_class_and_tiny_views(api, oracle):
             for f in ("shadow", "ao", "spec", "fetches"):
                 assert_bitwise(getattr(o, f).cpu().numpy(), getattr(p, f).cpu().numpy(), f"batch {counts} item {k} {f}")
         batch.close()
+
+
+@pytest.mark.parametrize("direction", [(1, 1, 1), (-1, 1, -1), (1, 1, 2), (2, -1, -1)])
+def test_ties_match_oracle(api, ctx, oracle, direction):
+    """Shadow rays from voxel centres along directions with equal components,
+    so that two or three t_max tie at every step (the diagonal steps, their
+    doubled and tripled best_t against maxt, and the step count), bit for bit
+    against the oracle, at the default and at a short maxt."""
+    from voxelengine_amd import configs, views
+    rng = np.random.default_rng(41)
+    dims = (40, 40, 40)
+    occ_h = ((rng.random(dims[::-1]) < 0.04).astype(np.uint8) *
+             rng.integers(1, 256, dims[::-1]).astype(np.uint8))
+    occ = api.Volume(ctx, *dims, nlevels=1)
+    occ.clear()
+    occ.upload_region(occ_h, [(0, 0, 0, *dims)])
+    occ.build_levels()
+    w, h = 32, 32
+    vd = views.scene_view((80, 80, 80), w, h)
+    g = api.GBuffer(w, h)
+    cells = rng.integers(2, 76, (w * h, 3)).astype(np.float32) + 0.5      # fine-voxel centres
+    g.pos.copy_(torch.from_numpy(cells))
+    g.normal.zero_()                                                     # the origin is the centre itself
+    g.material.fill_(20)
+    d = np.asarray(direction, np.float32)
+    d = (d / np.float32(np.sqrt(np.float32((d * d).sum())))).astype(np.float32)
+    pal = configs.make_palette()
+    for maxt in (128.0, 7.3):
+        params = configs.trace_params(n_shadow=1, n_ao=0, n_spec=0, shadow_maxt=maxt, march="dda", jitter_shadow=0)
+        params["light_dir"] = [float(x) for x in d]
+        vis = api.Visibility(w, h, fetches=True)
+        api.trace(ctx, occ, vd, g, torch.from_numpy(pal).cuda(), params, vis)
+        sh, _, _, fe = oracle.trace(occ_h, vd, params, cells, np.zeros((w * h, 3), np.float32),
+                                    np.full(w * h, 20, np.uint8), pal)
+        assert_bitwise(vis.shadow.cpu().numpy(), sh, f"shadow {direction} maxt {maxt}")
+        assert_bitwise(vis.fetches.cpu().numpy().astype(np.uint32), fe, f"fetches {direction} maxt {maxt}")
+        assert 0 < sh.sum() < sh.size
