@@ -206,7 +206,17 @@ VXD uint32_t spread_byte(uint32_t b) {
 // word 0 for those), so a march step carries no branch for it.
 VXD uint32_t texel_byte(const OccView& V, int32_t tx, int32_t ty, int32_t tz) {
     const bool in = (uint32_t)tx < (uint32_t)V.sx && (uint32_t)ty < (uint32_t)V.sy && (uint32_t)tz < (uint32_t)V.sz;
-    const size_t ci = in ? brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1) : 0;
+    // 32-bit index arithmetic unless the volume holds 2^32 cell words or more
+    const bool narrow = (uint64_t)(uint32_t)V.bx * (uint32_t)V.by * (uint32_t)V.bz < (1ull << 29);
+    size_t ci;
+    if (narrow)
+        ci = in ? (size_t)((((((uint32_t)(tz >> 2) * (uint32_t)V.by + (uint32_t)(ty >> 2)) * (uint32_t)V.bx +
+                             (uint32_t)(tx >> 2))
+                            << 3) +
+                           sub3(tx >> 1, ty >> 1, tz >> 1)))
+                : 0;
+    else
+        ci = in ? brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1) : 0;
     const uint64_t w = V.cells[ci];
     const uint32_t x = (uint32_t)(w >> texel_base(tx, ty, tz));
     const uint32_t b = (x & 3u) | ((x >> 2) & 0xcu) | ((x >> 12) & 0x30u) | ((x >> 14) & 0xc0u);
