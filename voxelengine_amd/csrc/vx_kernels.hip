@@ -425,10 +425,14 @@ VXD float sparse_march(const OccView& V, f3 o, f3 d, float dist, float step_fact
     while (dd < 16.0f) {
         it++;
         uint32_t vol = texel_byte(V, f2i(pos.x / 2.0f), f2i(pos.y / 2.0f), f2i(pos.z / 2.0f));
+        // mod(p, 0.5) > 0.25 (Light.frag:144-146): mod is p - 0.5 floor(2p),
+        // whose float value is half that of 2p - floor(2p) (doubling is
+        // exact), so the test is fract(2p) > 0.5. v_fract_f32 returns values
+        // that round to 1 as 1 - 2^-24, still > 0.5, and NaN for NaN.
         uint32_t bit = 0u;
-        bit += gmod(pos.x, 0.5f) > 0.25f ? 1u : 0u;
-        bit += gmod(pos.y, 0.5f) > 0.25f ? 2u : 0u;
-        bit += gmod(pos.z, 0.5f) > 0.25f ? 4u : 0u;
+        bit += __builtin_amdgcn_fractf(pos.x * 2.0f) > 0.5f ? 1u : 0u;
+        bit += __builtin_amdgcn_fractf(pos.y * 2.0f) > 0.5f ? 2u : 0u;
+        bit += __builtin_amdgcn_fractf(pos.z * 2.0f) > 0.5f ? 4u : 0u;
         if (((1u << bit) & vol) != 0u) {
             R.hit = 1; R.term = 1; R.steps = R.fetches = it; R.t = dd;
             return dd;
