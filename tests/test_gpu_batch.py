@@ -65,6 +65,15 @@ def test_batch_equals_plain_launches_and_oracle(ctx, oracle):
     assert_bitwise(outs[1].shadow.cpu().numpy(), sh, "oracle shadow")
     assert_bitwise(outs[1].spec.cpu().numpy(), sp, "oracle spec")
     batch.close()
+    # the same batch without fetch counts: same bits
+    outs2 = [api.Visibility(it["w"], it["h"]) for it in items]
+    batch2 = api.Batch(ctx, [(it["occ"], it["vd"], it["g"], o) for it, o in zip(items, outs2)], pal, params)
+    batch2.launch()
+    torch.cuda.synchronize()
+    for k, (o, p) in enumerate(zip(outs2, plain)):
+        for f in ("shadow", "ao", "spec"):
+            assert_bitwise(getattr(o, f).cpu().numpy(), getattr(p, f).cpu().numpy(), f"item {k} {f} without fetch counts")
+    batch2.close()
 
 
 def test_batch_rejects_reference_marchers(ctx):

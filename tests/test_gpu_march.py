@@ -42,6 +42,11 @@ def _trace_both(api, ctx, oracle, occ, occ_h, model, variant, params, w=72, h=52
                        pal)
     got = (vis.shadow.cpu().numpy(), vis.ao.cpu().numpy(), vis.spec.cpu().numpy(),
            vis.fetches.cpu().numpy().astype(np.uint32))
+    # the launch without fetch counts keeps a per-block step count: same bits
+    vis2 = api.Visibility(w, h)
+    api.trace(ctx, occ, vd, g, torch.from_numpy(pal).cuda(), params, vis2)
+    for name, a, b in zip(("shadow", "ao", "spec"), (vis2.shadow, vis2.ao, vis2.spec), ref):
+        assert_bitwise(a.cpu().numpy(), b, f"{name} without fetch counts")
     return got, ref
 
 

@@ -895,7 +895,10 @@ VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gx
 // marches one item at a time, so the item's grid address and pitches stay
 // wave-uniform: in a batch, a wave that draws a tile of another item first
 // lets its rays in flight finish, then switches.
-template <int BLOCK, int STEPS, bool SINGLE>
+// COUNT: every ray's step count is kept for the fetch-count output. Without
+// it the count only serves the 256-step limit, which is tested at block ends,
+// so it advances by STEPS once per block.
+template <int BLOCK, int STEPS, bool SINGLE, bool COUNT>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)))
 k_march(MarchArgs A) {
     static_assert(256 % STEPS == 0, "the step limit is tested at block ends");
@@ -1078,11 +1081,12 @@ k_march(MarchArgs A) {
                 R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
                 R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
                 R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
-                R.meta += 1u << 20;
                 // A ray enters the march between blocks of STEPS steps with a
                 // count of 0 and steps once per iteration while it runs, so its
                 // count reaches 256 (a multiple of STEPS) only at a block's last
                 // step: the limit is tested there alone.
+                if (COUNT) R.meta += 1u << 20;
+                else if (u == STEPS - 1) R.meta += (uint32_t)STEPS << 20;
                 running = !occ && (u != STEPS - 1 || (int32_t)R.meta >= 0) && bt < R.maxt;
                 stopped = occ;
                 // the word is fetched only when the ray enters another cell
@@ -1181,14 +1185,18 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
 // Launchers (template instances live in this file).
 template <int BLOCK, int STEPS>
 static void launch_march_t(bool single, uint32_t grid, hipStream_t s, const MarchArgs& A) {
-    if (single) hipLaunchKernelGGL((k_march<BLOCK, STEPS, true>), dim3(grid), dim3(BLOCK), 0, s, A);
-    else hipLaunchKernelGGL((k_march<BLOCK, STEPS, false>), dim3(grid), dim3(BLOCK), 0, s, A);
+    // the per-step count only where the fetch counts are asked for
+    const bool count = A.fres != nullptr;
+    if (single && count) hipLaunchKernelGGL((k_march<BLOCK, STEPS, true, true>), dim3(grid), dim3(BLOCK), 0, s, A);
+    else if (single) hipLaunchKernelGGL((k_march<BLOCK, STEPS, true, false>), dim3(grid), dim3(BLOCK), 0, s, A);
+    else if (count) hipLaunchKernelGGL((k_march<BLOCK, STEPS, false, true>), dim3(grid), dim3(BLOCK), 0, s, A);
+    else hipLaunchKernelGGL((k_march<BLOCK, STEPS, false, false>), dim3(grid), dim3(BLOCK), 0, s, A);
 }
 
 template <int BLOCK, int STEPS>
 static const void* march_kernel_t(bool single) {
-    return single ? reinterpret_cast<const void*>(&k_march<BLOCK, STEPS, true>)
-                  : reinterpret_cast<const void*>(&k_march<BLOCK, STEPS, false>);
+    return single ? reinterpret_cast<const void*>(&k_march<BLOCK, STEPS, true, false>)
+                  : reinterpret_cast<const void*>(&k_march<BLOCK, STEPS, false, false>);
 }
 
 const void* march_kernel(int block, int steps, bool single) {
