@@ -887,8 +887,13 @@ VXD uint32_t cell_shift(uint32_t x, uint32_t y, uint32_t z) {
 
 // Grid word index of the lane's voxel. It stays inside the grid: a ray starts
 // inside the bounds and stops at the first border voxel it reaches.
+// Two v_mad_u32_u24 with the pitches in scalar registers (the compiler's own
+// choice, two multiplies and a three-way add, is one instruction longer).
 VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gxy) {
-    return __umul24((uint32_t)(z >> 6), gxy) + (__umul24((uint32_t)(y >> 4), gx) + (uint32_t)(x >> 2));
+    uint32_t a, r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"((uint32_t)(y >> 4)), "s"(gx), "v"((uint32_t)(x >> 2)));
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"((uint32_t)(z >> 6)), "s"(gxy), "v"(a));
+    return r;
 }
 
 // Occupancy: 7 waves per SIMD (72 VGPRs, 140 KB of LDS rings per CU). A wave
@@ -1090,11 +1095,13 @@ k_march(MarchArgs A) {
                 running = !occ && (u != STEPS - 1 || (int32_t)R.meta >= 0) && bt < R.maxt;
                 stopped = occ;
                 // the word is fetched only when the ray enters another cell
-                const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
-                if (ci != R.ci) {
-                    R.ci = ci;
-                    R.cw = grid[ci];
-                }
+                uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
+                if (ci != R.ci) R.cw = grid[ci];
+                // R.ci takes ci's register on both paths: without the empty asm
+                // the compiler reuses R.ci's where they are equal and copies ci
+                // into it inside the branch, one move per step
+                asm("" : "+v"(ci));
+                R.ci = ci;
             }
         }
     }
