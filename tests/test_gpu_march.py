@@ -78,6 +78,30 @@ def test_march_volume_sizes_and_tunings(api, ctx, oracle, vol_dims):
         ctx.reset_tuning()
 
 
+def test_step_limit_in_every_tuning(api, ctx, oracle):
+    """Spec rays through a 320^3 volume holding only a floor: many end at the
+    256-step limit (one ray per pixel, so such a pixel counts 4 * 256
+    fetches). Every tuning's block length, with and without fetch counts,
+    matches the oracle."""
+    from voxelengine_amd import configs
+    model = _model(api, ctx, (96, 80, 88), seed=11)
+    S = (320, 320, 320)
+    host = np.zeros(S[::-1], np.uint8)
+    host[:, :2, :] = 9                       # a floor: y < 2
+    occ = api.Volume(ctx, *S, nlevels=1)
+    occ.upload_region(host, [(0, 0, 0, S[0], S[1], S[2])])
+    params = configs.trace_params(n_shadow=0, n_ao=0, n_spec=1, spec_maxt=2000.0, march="dda")
+    try:
+        for tu in TUNINGS:
+            ctx.set_tuning(**tu)
+            for variant in (0, 2):
+                got, ref = _trace_both(api, ctx, oracle, occ, host, model, variant, params)
+                _assert_same(got, ref, f"tuning {tu} view {variant}")
+                assert (ref[3] == 1024).sum() > 50
+    finally:
+        ctx.reset_tuning()
+
+
 def test_march_after_volume_edits(api, ctx, oracle):
     from voxelengine_amd import configs
     model = _model(api, ctx, (64, 64, 64), seed=12)
