@@ -939,6 +939,9 @@ k_march(MarchArgs A) {
         const uint64_t runm = __ballot(running);
         if (runm == 0 || (uint32_t)__popcll(~runm) >= (uint32_t)A.refill) {
             // ---- finish ended rays
+            // (set-up and refill run at the base wave priority, the steps one
+            // above it: the SIMD then favours waves that are marching)
+            __builtin_amdgcn_s_setprio(0);
             if (hasray && !running) {
                 const uint32_t kind = (R.meta >> 16) & 3u, n = (R.meta >> 20) - 0x700u - (stopped ? 1u : 0u);
                 const MarchItem& I = march_item_uniform<SINGLE>(A, citem);
@@ -1064,6 +1067,7 @@ k_march(MarchArgs A) {
             if (__ballot(hasray) == 0 && gen_done && head == tail) break;
         }
         // ---- march
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int u = 0; u < STEPS; ++u) {
             if (running) {
