@@ -114,6 +114,7 @@ def lib():
         L.vxo_primary.argtypes = [C.POINTER(P), C.POINTER(C.c_int32), C.POINTER(View), P, P, P, C.c_int32]
         L.vxo_trace.argtypes = [C.POINTER(Occ), C.POINTER(View), C.POINTER(TraceParams), P, P, P, P,
                                 P, P, P, P, P, C.c_int32, C.c_int32]
+        L.vxo_pixel_rays.argtypes = [C.POINTER(View), C.POINTER(TraceParams), P, P, P, P, P, C.c_int32, P]
         L.vxo_shade_f32.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, C.c_int32]
         L.vxo_shade_f64.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, C.c_int32]
         L.vxo_shade_backward_f64.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, P, P]
@@ -360,6 +361,19 @@ def trace(vol, vd: dict, params: dict, pos, nrm, mat, palette, pixels=None, nthr
     lib().vxo_trace(C.byref(O), C.byref(v), C.byref(tp), _p(pos), _p(nrm), _p(mat), _p(palette),
                     _p(shadow), _p(ao), _p(spec), _p(fetches), pp, npix, nthreads)
     return shadow, ao, spec, fetches
+
+
+def pixel_rays(vd: dict, params: dict, pos, nrm, mat, palette, pixels):
+    """The rays the trace marches for each listed pixel: (npixels, nrays, 7)
+    float32 (origin3, dir3, maxt), zeros for sky pixels."""
+    pixels = np.ascontiguousarray(pixels, np.int32)
+    nr = params["n_shadow"] + params["n_ao"] + params["n_spec"]
+    out = np.zeros((pixels.size, nr, 7), np.float32)
+    v = view_struct(vd); tp = trace_params(params)
+    lib().vxo_pixel_rays(C.byref(v), C.byref(tp), _p(np.ascontiguousarray(pos, np.float32)),
+                         _p(np.ascontiguousarray(nrm, np.float32)), _p(np.ascontiguousarray(mat, np.uint8)),
+                         _p(np.ascontiguousarray(palette, np.float32)), _p(pixels), pixels.size, _p(out))
+    return out
 
 
 def shade_f32(vd, pos, nrm, mat, shadow, ao, palette, light, ssao=None, nthreads=0):

@@ -882,18 +882,14 @@ static inline float march_dist(const vxo_occ* v, int32_t march, const float o[3]
     return r.t;
 }
 
-/* One pixel of the ambient pass (LightAmbient.frag:134-175) and the specular
- * occlusion ray (LightReflection.frag:86-113), generalised to n_ao cosine AO
- * rays with a length cap and a GGX-sampled reflection ray (build-defined). */
-static void trace_pixel(const vxo_occ* v, const vxo_view* vw, const vxo_trace_params* p,
-                        const float* pos, const float* nrm, const uint8_t* mat, const float* palette,
-                        size_t i, uint8_t* shadow, float* ao, float* spec, uint32_t* fetches) {
-    uint32_t nf = 0;
-    if (mat[i] == 0) {
-        shadow[i] = 0; ao[i] = 0.0f; spec[i] = 0.0f;
-        if (fetches) fetches[i] = 0;
-        return;
-    }
+/* Rays of one non-sky pixel, in ray order: n_shadow shadow rays, n_ao AO rays,
+ * n_spec specular-occlusion rays; rays[7*r..7*r+6] = origin xyz, direction
+ * xyz, maxt. The ambient pass's ray set-up (LightAmbient.frag:134-175)
+ * generalised to n_ao cosine AO rays with a length cap, and a GGX-sampled
+ * reflection ray from the origin of LightReflection.frag:93-113
+ * (build-defined direction). */
+static void pixel_rays(const vxo_view* vw, const vxo_trace_params* p, const float* pos, const float* nrm,
+                       const uint8_t* mat, const float* palette, size_t i, float* rays) {
     const float* pp = pos + 3 * i;
     const float* n = nrm + 3 * i;
     uint32_t pk = vxo_pixkey(p->seed, p->frame, (uint32_t)i);
@@ -925,37 +921,31 @@ static void trace_pixel(const vxo_occ* v, const vxo_view* vw, const vxo_trace_pa
     float org[3];
     for (int k = 0; k < 3; ++k) org[k] = wcp[k] + n[k] * bias;
 
-    uint8_t lit = 1;
-    if (p->n_shadow > 0) {
-        int32_t h;
-        float t = march_dist(v, p->shadow_march, org, wd, p->shadow_maxt, &nf, &h);
-        if (p->shadow_march == VXO_MARCH_DDA) lit = h ? 0 : 1;
-        else lit = (t != p->shadow_maxt) ? 0 : 1;      /* LightAmbient.frag:167 */
+    float* r = rays;
+    for (int32_t s = 0; s < p->n_shadow; ++s, r += 7) {
+        r[0] = org[0]; r[1] = org[1]; r[2] = org[2];
+        r[3] = wd[0]; r[4] = wd[1]; r[5] = wd[2];
+        r[6] = p->shadow_maxt;
     }
-    shadow[i] = lit;
 
     /* AO rays, calculateAmbientIrradiance LightAmbient.frag:111-126 */
-    float aosum = 0.0f;
     if (p->n_ao > 0) {
         float T[3], B[3];
         tangent_frame(n, T, B);
-        for (int32_t a = 0; a < p->n_ao; ++a) {
+        for (int32_t a = 0; a < p->n_ao; ++a, r += 7) {
             float u1 = vxo_rand(pk, (uint32_t)(4 * (1 + a) + 0));
             float u2 = vxo_rand(pk, (uint32_t)(4 * (1 + a) + 1));
-            float rv[3], dir[3];
+            float rv[3];
             vxo_cosine_hemisphere(u1, u2, rv);
-            for (int k = 0; k < 3; ++k) dir[k] = (T[k] * rv[0] + B[k] * rv[1]) + n[k] * rv[2];
-            int32_t h;
-            float t = march_dist(v, p->ao_march, org, dir, p->ao_maxt, &nf, &h);
-            float q = g_min(t, p->ao_maxt) / p->ao_maxt;
-            aosum = aosum + q * q;
+            r[0] = org[0]; r[1] = org[1]; r[2] = org[2];
+            for (int k = 0; k < 3; ++k) r[3 + k] = (T[k] * rv[0] + B[k] * rv[1]) + n[k] * rv[2];
+            r[6] = p->ao_maxt;
         }
-        ao[i] = aosum / (float)p->n_ao;
-    } else {
-        ao[i] = 1.0f;
     }
 
-    /* specular-occlusion ray, LightReflection.frag:86-113 with GGX sampling */
+    /* specular-occlusion ray: LightReflection.frag:93,113 origin (wcp + normal
+     * * noise.w, marched from + normal) with a GGX-sampled half vector
+     * (build-defined; DistributionGGX's NDF, PBR.frag:12-23) */
     if (p->n_spec > 0) {
         uint32_t base = (uint32_t)(4 * (1 + p->n_ao));
         float u1 = vxo_rand(pk, base + 0), u2 = vxo_rand(pk, base + 1), u3 = vxo_rand(pk, base + 3);
@@ -977,11 +967,68 @@ static void trace_pixel(const vxo_occ* v, const vxo_view* vw, const vxo_trace_pa
         for (int k = 0; k < 3; ++k) h[k] = (T[k] * hl[0] + B[k] * hl[1]) + n[k] * hl[2];
         float I[3] = {-Vd[0], -Vd[1], -Vd[2]};
         float dhi = dot3(h, I);
-        float wdir[3], so[3];
-        for (int k = 0; k < 3; ++k) wdir[k] = I[k] - (2.0f * dhi) * h[k];
-        for (int k = 0; k < 3; ++k) so[k] = (pp[k] + n[k] * u3) + n[k];
+        for (int k = 0; k < 3; ++k) r[3 + k] = I[k] - (2.0f * dhi) * h[k];
+        for (int k = 0; k < 3; ++k) r[k] = (pp[k] + n[k] * u3) + n[k];
+        r[6] = p->spec_maxt;
+    }
+}
+
+void vxo_pixel_rays(const vxo_view* vw, const vxo_trace_params* p, const float* pos, const float* nrm,
+                    const uint8_t* mat, const float* palette, const int32_t* pixels, int32_t npixels,
+                    float* rays) {
+    const int32_t nr = p->n_shadow + p->n_ao + p->n_spec;
+    for (int32_t j = 0; j < npixels; ++j) {
+        const size_t i = (size_t)pixels[j];
+        float* r = rays + (size_t)j * nr * 7;
+        if (mat[i] == 0) memset(r, 0, (size_t)nr * 7 * sizeof(float));
+        else pixel_rays(vw, p, pos, nrm, mat, palette, i, r);
+    }
+}
+
+/* One pixel of the ambient pass (LightAmbient.frag:134-175) and the specular
+ * occlusion ray (LightReflection.frag:86-113): the pixel's rays, each
+ * marched, folded in ray order. Shadow: lit unless the ray hit (DDA) or the
+ * marcher returned less than maxt (LightAmbient.frag:167); AO: mean of
+ * (min(t, cap) / cap)^2 (the d*d of LightAmbient.frag:121-125 per ray);
+ * spec: min(t, maxt) / maxt. */
+static void trace_pixel(const vxo_occ* v, const vxo_view* vw, const vxo_trace_params* p,
+                        const float* pos, const float* nrm, const uint8_t* mat, const float* palette,
+                        size_t i, uint8_t* shadow, float* ao, float* spec, uint32_t* fetches) {
+    uint32_t nf = 0;
+    if (mat[i] == 0) {
+        shadow[i] = 0; ao[i] = 0.0f; spec[i] = 0.0f;
+        if (fetches) fetches[i] = 0;
+        return;
+    }
+    float rays[7 * (1 + 64 + 1)];
+    pixel_rays(vw, p, pos, nrm, mat, palette, i, rays);
+    const float* r = rays;
+
+    uint8_t lit = 1;
+    for (int32_t s = 0; s < p->n_shadow; ++s, r += 7) {
+        int32_t h;
+        float t = march_dist(v, p->shadow_march, r, r + 3, r[6], &nf, &h);
+        if (p->shadow_march == VXO_MARCH_DDA) lit = h ? 0 : 1;
+        else lit = (t != p->shadow_maxt) ? 0 : 1;      /* LightAmbient.frag:167 */
+    }
+    shadow[i] = lit;
+
+    if (p->n_ao > 0) {
+        float aosum = 0.0f;
+        for (int32_t a = 0; a < p->n_ao; ++a, r += 7) {
+            int32_t h;
+            float t = march_dist(v, p->ao_march, r, r + 3, r[6], &nf, &h);
+            float q = g_min(t, p->ao_maxt) / p->ao_maxt;
+            aosum = aosum + q * q;
+        }
+        ao[i] = aosum / (float)p->n_ao;
+    } else {
+        ao[i] = 1.0f;
+    }
+
+    if (p->n_spec > 0) {
         int32_t hh;
-        float t = march_dist(v, p->spec_march, so, wdir, p->spec_maxt, &nf, &hh);
+        float t = march_dist(v, p->spec_march, r, r + 3, r[6], &nf, &hh);
         spec[i] = g_min(t, p->spec_maxt) / p->spec_maxt;
     } else {
         spec[i] = 1.0f;

@@ -151,6 +151,12 @@ void vxo_light_taa(int32_t width, int32_t height, int32_t frame, uint32_t seed, 
 /* The TAA sampler: bilinear, 8-bit sub-texel weights, opaque-black border,
  * over an n-channel float image (test hook). */
 void vxo_taa_sample(const float* img, int32_t width, int32_t height, int32_t n, float u, float v, float* out);
+/* The rays vxo_trace marches for each listed pixel, in ray order (n_shadow
+ * shadow, n_ao AO, n_spec spec): npixels * nrays * 7 floats (origin3, dir3,
+ * maxt); all zero for a sky pixel (material 0). */
+void vxo_pixel_rays(const vxo_view* view, const vxo_trace_params* p, const float* pos, const float* nrm,
+                    const uint8_t* mat, const float* palette, const int32_t* pixels, int32_t npixels,
+                    float* rays);
 /* pixels: optional list of pixel indices to process (NULL = all). */
 void vxo_trace(const vxo_occ* v, const vxo_view* view, const vxo_trace_params* p,
                const float* pos, const float* nrm, const uint8_t* mat, const float* palette,
