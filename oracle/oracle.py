@@ -62,6 +62,10 @@ class Light(C.Structure):
                 ("angle_attenuation", C.c_float), ("pad", C.c_int32)]
 
 
+class ReflectionParams(C.Structure):
+    _fields_ = [("march", C.c_int32), ("maxt", C.c_float), ("seed", C.c_uint32), ("frame", C.c_int32)]
+
+
 class LightParams(C.Structure):
     _fields_ = [("point_march", C.c_int32), ("spot_march", C.c_int32), ("seed", C.c_uint32), ("frame", C.c_int32)]
 
@@ -112,9 +116,17 @@ def lib():
                                            C.c_int32, f32p, f32p, C.POINTER(C.c_uint8)]
         L.vxo_intersect_volume.restype = C.c_int32
         L.vxo_primary.argtypes = [C.POINTER(P), C.POINTER(C.c_int32), C.POINTER(View), P, P, P, C.c_int32]
+        L.vxo_primary_pixels.argtypes = [C.POINTER(P), C.POINTER(C.c_int32), C.POINTER(View), P, C.c_int32, P, P, P,
+                                         C.c_int32]
         L.vxo_trace.argtypes = [C.POINTER(Occ), C.POINTER(View), C.POINTER(TraceParams), P, P, P, P,
                                 P, P, P, P, P, C.c_int32, C.c_int32]
         L.vxo_pixel_rays.argtypes = [C.POINTER(View), C.POINTER(TraceParams), P, P, P, P, P, C.c_int32, P]
+        L.vxo_sky_procedural.argtypes = [f32p, f32p]
+        L.vxo_cube_sample.argtypes = [P, C.c_int32, f32p, f32p]
+        L.vxo_reflection_rays.argtypes = [C.POINTER(View), C.POINTER(ReflectionParams), P, P, P, P, P, C.c_int32, P]
+        L.vxo_reflection.argtypes = [C.POINTER(Occ), C.POINTER(View), C.POINTER(ReflectionParams), P, P, P, P, P, P,
+                                     P, C.c_int32, P, P, C.c_int32]
+        L.vxo_ssao.argtypes = [C.POINTER(View), P, P, P, C.c_uint32, P, C.c_int32]
         L.vxo_shade_f32.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, C.c_int32]
         L.vxo_shade_f64.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, C.c_int32]
         L.vxo_shade_backward_f64.argtypes = [C.POINTER(View), P, P, P, P, P, P, P, P, P, P, P]
@@ -274,8 +286,17 @@ def view_struct(vd: dict) -> View:
     return v
 
 
-def primary(mips, vd: dict, nthreads=0):
+def primary(mips, vd: dict, nthreads=0, pixels=None):
+    """G-buffer of the view: (pos, nrm, mat) for every pixel, or compact for the
+    listed pixels."""
     ptrs, dims = _mips_args(mips)
+    if pixels is not None:
+        pixels = np.ascontiguousarray(pixels, np.int32)
+        n = pixels.size
+        pos = np.zeros((n, 3), np.float32); nrm = np.zeros((n, 3), np.float32); mat = np.zeros(n, np.uint8)
+        v = view_struct(vd)
+        lib().vxo_primary_pixels(ptrs, dims, C.byref(v), _p(pixels), n, _p(pos), _p(nrm), _p(mat), nthreads)
+        return pos, nrm, mat
     W, H = vd["width"], vd["height"]
     pos = np.zeros((H * W, 3), np.float32); nrm = np.zeros((H * W, 3), np.float32)
     mat = np.zeros(H * W, np.uint8)
@@ -438,3 +459,60 @@ def light_pass(vol, vd: dict, lights: np.ndarray, lp: LightParams, pos, nrm, mat
     lib().vxo_light_pass(C.byref(O), C.byref(v), lights.ctypes.data, lights.size, C.byref(lp), _p(pos), _p(nrm),
                          _p(mat), _p(np.ascontiguousarray(palette, np.float32)), _p(rgb), _p(fetches), nthreads)
     return rgb, fetches
+
+
+# ---------------------------------------------------------------- reflection pass
+def reflection_params(march=1, maxt=256.0, seed=0, frame=0) -> ReflectionParams:
+    return ReflectionParams(march, float(maxt), seed & 0xFFFFFFFF, frame)
+
+
+def sky_procedural(e):
+    out = (C.c_float * 3)()
+    lib().vxo_sky_procedural(_f3(e), out)
+    return np.array(out[:], np.float32)
+
+
+def cube_sample(faces, d):
+    """faces: (6, size, size, 4) float32."""
+    faces = np.ascontiguousarray(faces, np.float32)
+    out = (C.c_float * 4)()
+    lib().vxo_cube_sample(_p(faces), faces.shape[1], _f3(d), out)
+    return np.array(out[:], np.float32)
+
+
+def reflection_rays(vd, rp: ReflectionParams, pos, nrm, mat, palette, pixels):
+    """(npixels, 6): wcp (fine units) and direction of each listed pixel's reflection ray."""
+    pixels = np.ascontiguousarray(pixels, np.int32)
+    out = np.zeros((pixels.size, 6), np.float32)
+    v = view_struct(vd)
+    lib().vxo_reflection_rays(C.byref(v), C.byref(rp), _p(np.ascontiguousarray(pos, np.float32)),
+                              _p(np.ascontiguousarray(nrm, np.float32)), _p(np.ascontiguousarray(mat, np.uint8)),
+                              _p(np.ascontiguousarray(palette, np.float32)), _p(pixels), pixels.size, _p(out))
+    return out
+
+
+def reflection(vol, vd, rp: ReflectionParams, pos, nrm, mat, palette, depth, light, sky=None, nthreads=0):
+    """LightReflection over numpy buffers: (out (npx, 4), t (npx,))."""
+    npx = vd["width"] * vd["height"]
+    out = np.zeros((npx, 4), np.float32)
+    t = np.zeros(npx, np.float32)
+    O = occ(vol); v = view_struct(vd)
+    sky_a = None if sky is None else np.ascontiguousarray(sky, np.float32)
+    lib().vxo_reflection(C.byref(O), C.byref(v), C.byref(rp), _p(np.ascontiguousarray(pos, np.float32)),
+                         _p(np.ascontiguousarray(nrm, np.float32)), _p(np.ascontiguousarray(mat, np.uint8)),
+                         _p(np.ascontiguousarray(palette, np.float32)),
+                         _p(np.ascontiguousarray(depth, np.float32).reshape(npx)),
+                         _p(np.ascontiguousarray(light, np.float32).reshape(npx, 4)),
+                         _p(sky_a) if sky_a is not None else None, 0 if sky_a is None else sky_a.shape[1],
+                         _p(out), _p(t), nthreads)
+    return out, t
+
+
+def ssao(vd, nrm, mat, depth, seed=0, nthreads=0):
+    """calculateOcclusion per pixel (npx,), 1 where nothing was hit."""
+    npx = vd["width"] * vd["height"]
+    out = np.zeros(npx, np.float32)
+    v = view_struct(vd)
+    lib().vxo_ssao(C.byref(v), _p(np.ascontiguousarray(nrm, np.float32)), _p(np.ascontiguousarray(mat, np.uint8)),
+                   _p(np.ascontiguousarray(depth, np.float32).reshape(npx)), seed & 0xFFFFFFFF, _p(out), nthreads)
+    return out

@@ -506,6 +506,20 @@ static void pixel_ray(const vxo_view* vw, int32_t px, int32_t py, float dir[3], 
 /* GeometryVoxel main (GeometryVoxel.frag:127-183) for the world model with an
  * identity WorldMatrix: localCameraPos = CameraPosition*10
  * (GeometryVoxel.vert:74); G-buffer gets the fp32 local hit point. */
+static void primary_pixel(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw, int32_t px,
+                          int32_t py, float* pos, float* nrm, uint8_t* mat) {
+    float dir[3], ndc[2], cam[3], start[3], hit[3] = {0, 0, 0}, n[3] = {0, 0, 0};
+    pixel_ray(vw, px, py, dir, ndc);
+    for (int k = 0; k < 3; ++k) cam[k] = vw->cam_pos[k] * 10.0f;
+    float size[3] = {(float)dims[0], (float)dims[1], (float)dims[2]};
+    vxo_clip_to_aabb(cam, dir, size, start);
+    uint8_t m = 0;
+    int h = vxo_intersect_volume(mips, dims, start, dir, cam, ndc, vw->res, vw->frame, hit, n, &m);
+    if (!h) m = 0;
+    for (int k = 0; k < 3; ++k) { pos[k] = h ? hit[k] : 0.0f; nrm[k] = h ? n[k] : 0.0f; }
+    *mat = m;
+}
+
 void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw,
                  float* pos, float* nrm, uint8_t* mat, int32_t nthreads) {
     const int32_t W = vw->width, H = vw->height;
@@ -516,17 +530,23 @@ void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_
     for (int32_t py = 0; py < H; ++py) {
         for (int32_t px = 0; px < W; ++px) {
             size_t i = (size_t)py * W + px;
-            float dir[3], ndc[2], cam[3], start[3], hit[3] = {0, 0, 0}, n[3] = {0, 0, 0};
-            pixel_ray(vw, px, py, dir, ndc);
-            for (int k = 0; k < 3; ++k) cam[k] = vw->cam_pos[k] * 10.0f;
-            float size[3] = {(float)dims[0], (float)dims[1], (float)dims[2]};
-            vxo_clip_to_aabb(cam, dir, size, start);
-            uint8_t m = 0;
-            int h = vxo_intersect_volume(mips, dims, start, dir, cam, ndc, vw->res, vw->frame, hit, n, &m);
-            if (!h) m = 0;
-            for (int k = 0; k < 3; ++k) { pos[3 * i + k] = h ? hit[k] : 0.0f; nrm[3 * i + k] = h ? n[k] : 0.0f; }
-            mat[i] = m;
+            primary_pixel(mips, dims, vw, px, py, pos + 3 * i, nrm + 3 * i, mat + i);
         }
+    }
+    (void)nthreads;
+}
+
+void vxo_primary_pixels(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw,
+                        const int32_t* pixels, int32_t npixels, float* pos, float* nrm, uint8_t* mat,
+                        int32_t nthreads) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+    for (int32_t j = 0; j < npixels; ++j) {
+        const int32_t i = pixels[j];
+        primary_pixel(mips, dims, vw, i % vw->width, i / vw->width, pos + 3 * (size_t)j, nrm + 3 * (size_t)j,
+                      mat + j);
     }
     (void)nthreads;
 }
@@ -977,6 +997,9 @@ void vxo_pixel_rays(const vxo_view* vw, const vxo_trace_params* p, const float* 
                     const uint8_t* mat, const float* palette, const int32_t* pixels, int32_t npixels,
                     float* rays) {
     const int32_t nr = p->n_shadow + p->n_ao + p->n_spec;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int32_t j = 0; j < npixels; ++j) {
         const size_t i = (size_t)pixels[j];
         float* r = rays + (size_t)j * nr * 7;
@@ -1381,6 +1404,260 @@ void vxo_light_pass(const vxo_occ* v, const vxo_view* vw, const vxo_light* light
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
 #endif
     for (int32_t i = 0; i < npx; ++i) light_pixel(v, vw, lights, nlights, p, pos, nrm, mat, palette, rgb, fetches, i);
+    (void)nthreads;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reflection pass: LightReflection.frag:60-139 (LightReflectionPipeline.h:34) */
+/* ------------------------------------------------------------------------ */
+
+/* getSkyColor of LightAmbient.frag:128-131 (the formula LightReflection.frag:
+ * 55-56 keeps commented out beside its skybox lookup): pow(x, 2) = x*x. */
+void vxo_sky_procedural(const float e[3], float out[3]) {
+    float ey = (g_max(e[1], 0.0f) * 0.8f + 0.2f) * 0.8f;
+    float a = 1.0f - ey;
+    out[0] = (a * a) * 1.1f;
+    out[1] = a * 1.1f;
+    out[2] = (0.6f + a * 0.4f) * 1.1f;
+}
+
+/* texture(samplerCube, d): the face of the major axis (x, then y, then z on
+ * ties) and its (s, t) per the Vulkan cube-map face selection table, then
+ * the build's bilinear filter (8-bit sub-texel weights) with texel
+ * coordinates clamped to the face (no filtering across face edges). Faces
+ * +X, -X, +Y, -Y, +Z, -Z, each size x size RGBA, rows from t = 0. */
+void vxo_cube_sample(const float* faces, int32_t size, const float d[3], float out[4]) {
+    float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+    int face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d[0] >= 0.0f) { face = 0; sc = -d[2]; tc = -d[1]; }
+        else { face = 1; sc = d[2]; tc = -d[1]; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d[1] >= 0.0f) { face = 2; sc = d[0]; tc = d[2]; }
+        else { face = 3; sc = d[0]; tc = -d[2]; }
+    } else {
+        ma = az;
+        if (d[2] >= 0.0f) { face = 4; sc = d[0]; tc = -d[1]; }
+        else { face = 5; sc = -d[0]; tc = -d[1]; }
+    }
+    float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
+    const float fs = (float)size;
+    float x = s * fs - 0.5f, y = t * fs - 0.5f;
+    x = g_clamp(x, 0.0f, fs - 1.0f);
+    y = g_clamp(y, 0.0f, fs - 1.0f);
+    int32_t x0, y0;
+    float fx, fy;
+    taa_weight(x, size, &x0, &fx);
+    taa_weight(y, size, &y0, &fy);
+    int32_t x1 = x0 + 1 < size ? x0 + 1 : size - 1, y1 = y0 + 1 < size ? y0 + 1 : size - 1;
+    const float* F = faces + (size_t)face * size * size * 4;
+    for (int c = 0; c < 4; ++c) {
+        float a = F[((size_t)y0 * size + x0) * 4 + c], b = F[((size_t)y0 * size + x1) * 4 + c];
+        float cc = F[((size_t)y1 * size + x0) * 4 + c], dd = F[((size_t)y1 * size + x1) * 4 + c];
+        out[c] = (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (cc * (1.0f - fx) + dd * fx) * fy;
+    }
+}
+
+/* The reflection ray of pixel i (LightReflection.frag:79-98), in world space:
+ * wcp = the hit (fine units) + normal * noise.w, direction
+ * mix(normalize(reflect(-V, N)), cosineSampleHemisphere(noise.xy) with its z
+ * signed by noise.z, roughness * 0.1) * (1 + noise.w * 0.5); the march starts
+ * at wcp + normal. noise = the pixel's four counter-hash values (the same
+ * stream as the ambient pass's shadow jitter, as both passes read the same
+ * blue-noise texel). */
+static void reflection_ray(const vxo_view* vw, const vxo_reflection_params* p, const float* pos, const float* nrm,
+                           const uint8_t* mat, const float* palette, size_t i, float wcp[3], float wd[3]) {
+    const float* pp = pos + 3 * i;
+    const float* n = nrm + 3 * i;
+    float rough = palette[6 * mat[i] + 3];
+    uint32_t pk = vxo_pixkey(p->seed, p->frame, (uint32_t)i);
+    float nz[4];
+    for (int k = 0; k < 4; ++k) nz[k] = vxo_rand(pk, (uint32_t)k);
+    float cam[3], vv[3], V[3], I[3], R[3];
+    for (int k = 0; k < 3; ++k) cam[k] = vw->cam_pos[k] * 10.0f;
+    for (int k = 0; k < 3; ++k) vv[k] = cam[k] - pp[k];
+    normalize3(vv, V);
+    for (int k = 0; k < 3; ++k) I[k] = -V[k];
+    float dni = dot3(n, I);
+    for (int k = 0; k < 3; ++k) R[k] = I[k] - (2.0f * dni) * n[k];     /* reflect(-V, N) */
+    normalize3(R, wd);
+    float rv[3];
+    vxo_cosine_hemisphere(nz[0], nz[1], rv);
+    rv[2] = rv[2] * vxo_sign(nz[2] - 0.5f);
+    const float a = rough * 0.1f;
+    for (int k = 0; k < 3; ++k) wd[k] = wd[k] * (1.0f - a) + rv[k] * a;
+    for (int k = 0; k < 3; ++k) wcp[k] = pp[k] + n[k] * nz[3];
+    for (int k = 0; k < 3; ++k) wd[k] = wd[k] * (1.0f + nz[3] * 0.5f);
+}
+
+void vxo_reflection_rays(const vxo_view* vw, const vxo_reflection_params* p, const float* pos, const float* nrm,
+                         const uint8_t* mat, const float* palette, const int32_t* pixels, int32_t npixels,
+                         float* rays) {
+    for (int32_t j = 0; j < npixels; ++j)
+        reflection_ray(vw, p, pos, nrm, mat, palette, (size_t)pixels[j], rays + 6 * (size_t)j, rays + 6 * (size_t)j + 3);
+}
+
+static void reflection_pixel(const vxo_occ* v, const vxo_view* vw, const vxo_reflection_params* p,
+                             const float* pos, const float* nrm, const uint8_t* mat, const float* palette,
+                             const float* depth_img, const float* light, const float* sky, int32_t sky_size,
+                             size_t i, float* out, float* t_out) {
+    const int32_t W = vw->width, H = vw->height;
+    const float* pp = pos + 3 * i;
+    const float* n = nrm + 3 * i;
+    const float depth = depth_img[i];
+    const float rough = palette[6 * mat[i] + 3], metal = palette[6 * mat[i] + 4];
+    /* F0 = mix(vec3(0.04), albedo = 1, metallic); F = fresnelSchlickRoughness(max(N.V, 0), F0, roughness) * 5
+     * (LightReflection.frag:74-86, PBR.frag:8-10) */
+    const float f0 = 0.04f * (1.0f - metal) + 1.0f * metal;
+    float cam[3], vv[3], V[3];
+    for (int k = 0; k < 3; ++k) cam[k] = vw->cam_pos[k] * 10.0f;
+    for (int k = 0; k < 3; ++k) vv[k] = cam[k] - pp[k];
+    normalize3(vv, V);
+    const float ndv = g_max(dot3(n, V), 0.0f);
+    const float x = g_max(1.0f - ndv, 0.0f);
+    const float s5 = (((x * x) * x) * x) * x;
+    const float F = (f0 + (g_max(1.0f - rough, f0) - f0) * s5) * 5.0f + 0.0f;
+    float amb[3] = {0.0f, 0.0f, 0.0f};
+    float tt = NAN;
+    if (depth < 0.999f) {
+        float wcp[3], wd[3], o[3];
+        reflection_ray(vw, p, pos, nrm, mat, palette, i, wcp, wd);
+        for (int k = 0; k < 3; ++k) o[k] = wcp[k] + n[k];
+        uint32_t nf = 0;
+        int32_t hit;
+        const float t = march_dist(v, p->march, o, wd, p->maxt, &nf, &hit);
+        tt = t;
+        /* reprojection into the light buffer, LightReflection.frag:115-133 */
+        float h4[4], pv[4], pc[4];
+        for (int k = 0; k < 3; ++k) h4[k] = (wcp[k] + wd[k] * t) * 0.1f;
+        h4[3] = 1.0f;
+        mat_vec(vw->view, h4, pv);
+        pv[3] = 1.0f;
+        mat_vec(vw->proj, pv, pc);
+        const float lin = (pc[3] - NEAR_PLANE) / (FAR_PLANE - NEAR_PLANE);
+        const float u = ((pc[0] / pc[3]) * 1.0f) * 0.5f + 0.5f;
+        const float vv2 = ((pc[1] / pc[3]) * -1.0f) * 0.5f + 0.5f;
+        float ds;
+        taa_tex dt = {W, H, 0, 1, depth_img, NULL, NULL};
+        taa_sample(&dt, u, vv2, 1, &ds);
+        if (t == p->maxt) {
+            if (sky) {
+                float c4[4];
+                vxo_cube_sample(sky, sky_size, wd, c4);
+                amb[0] = c4[0]; amb[1] = c4[1]; amb[2] = c4[2];
+            } else {
+                vxo_sky_procedural(wd, amb);
+            }
+        } else if (lin > ds - 0.001f && lin < ds + 0.001f) {
+            float l4[4];
+            taa_tex lt = {W, H, 0, 4, light, NULL, NULL};
+            taa_sample(&lt, u, vv2, 4, l4);
+            amb[0] = l4[0]; amb[1] = l4[1]; amb[2] = l4[2];
+        }
+    }
+    for (int k = 0; k < 3; ++k) out[4 * i + k] = (amb[k] * F) * (1.0f - rough);
+    out[4 * i + 3] = F;
+    if (t_out) t_out[i] = tt;
+}
+
+void vxo_reflection(const vxo_occ* v, const vxo_view* vw, const vxo_reflection_params* p, const float* pos,
+                    const float* nrm, const uint8_t* mat, const float* palette, const float* depth,
+                    const float* light, const float* sky, int32_t sky_size, float* out, float* t_out,
+                    int32_t nthreads) {
+    const int32_t npx = vw->width * vw->height;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+    for (int32_t i = 0; i < npx; ++i)
+        reflection_pixel(v, vw, p, pos, nrm, mat, palette, depth, light, sky, sky_size, (size_t)i, out, t_out);
+    (void)nthreads;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Screen-space occlusion: calculateOcclusion / screenspaceOcclusion,        */
+/* LightAmbient.frag:59-79,89-109                                            */
+/* ------------------------------------------------------------------------ */
+static const float OCC_STRENGTH = 3.0f;     /* LightAmbient.frag:9 */
+static const float OCC_SIZE = 0.2f;         /* :10 */
+static const float OCC_THICKNESS = 0.2f;    /* :11 */
+
+/* The sample's noise (getNoise(i), LightAmbient.frag:44-47): the pixel's
+ * counter hash under its own key, indices 4 i + {0, 1, 2}. */
+static uint32_t ssao_key(uint32_t seed, int32_t frame, uint32_t pixel) {
+    return vxo_pixkey(seed ^ 0x632be5abu, frame, pixel);
+}
+
+static float ssao_pixel(const vxo_view* vw, const float* nrm, const float* depth_img, uint32_t seed, size_t i) {
+    const int32_t W = vw->width, H = vw->height;
+    const int32_t px = (int32_t)(i % (size_t)W), py = (int32_t)(i / (size_t)W);
+    /* the view-space normal: (View * vec4(normal, 0)).xyz (LightAmbient.frag:192) */
+    float n4[4] = {nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.0f}, N[4];
+    mat_vec(vw->view, n4, N);
+    float t0[3], T[3], B[3], cb[3];
+    if (fabsf(N[2]) > 0.5f) { t0[0] = 0.0f; t0[1] = -N[2]; t0[2] = N[1]; }
+    else { t0[0] = -N[1]; t0[1] = N[0]; t0[2] = 0.0f; }
+    normalize3(t0, T);
+    cb[0] = N[1] * T[2] - N[2] * T[1];
+    cb[1] = N[2] * T[0] - N[0] * T[2];
+    cb[2] = N[0] * T[1] - N[1] * T[0];
+    normalize3(cb, B);
+    /* In.FarVec at the pixel centre (LightAmbient.vert:32-36, 43) */
+    const float u = ((float)px + 0.5f) / vw->res[0], v = ((float)py + 0.5f) / vw->res[1];
+    float c4[4] = {u * 2.0f - 1.0f, 1.0f - v * 2.0f, 1.0f, 1.0f}, f4[4];
+    mat_vec(vw->inv_proj, c4, f4);
+    const float fv[3] = {f4[0] / f4[3], f4[1] / f4[3], f4[2] / f4[3]};
+    const float depth = depth_img[i];
+    const float sc = depth * (1.0f + 1.0f / FAR_PLANE) + NEAR_PLANE / FAR_PLANE;
+    const float pos[3] = {fv[0] * sc, fv[1] * sc, fv[2] * sc};
+    const float size = OCC_SIZE * (1.0f + depth * 0.0f);
+    const uint32_t pk = ssao_key(seed, vw->frame, (uint32_t)i);
+    taa_tex dt = {W, H, 0, 1, depth_img, NULL, NULL};
+    float occ = 0.0f;
+    for (int s = 0; s < 4; ++s) {
+        const float nx = vxo_rand(pk, (uint32_t)(4 * s)), ny = vxo_rand(pk, (uint32_t)(4 * s + 1));
+        const float dist = vxo_rand(pk, (uint32_t)(4 * s + 2));
+        float rv[3], dir[3], dn[3];
+        vxo_cosine_hemisphere(nx, ny, rv);
+        for (int k = 0; k < 3; ++k) dir[k] = (T[k] * rv[0] + B[k] * rv[1]) + N[k] * rv[2];
+        normalize3(dir, dn);
+        /* screenspaceOcclusion(pos, normalize(dir) * size, dist) */
+        float mid[4];
+        for (int k = 0; k < 3; ++k) mid[k] = pos[k] + (dn[k] * size) * dist;
+        mid[3] = 1.0f;
+        float mp[4];
+        mat_vec(vw->proj, mid, mp);
+        const float sd = (mp[3] - NEAR_PLANE) / (FAR_PLANE - NEAR_PLANE);
+        const float su = ((mp[0] / mp[3]) * 1.0f) * 0.5f + 0.5f;
+        const float sv = ((mp[1] / mp[3]) * -1.0f) * 0.5f + 0.5f;
+        float mn;
+        taa_sample(&dt, su, sv, 1, &mn);
+        const float mx = mn + OCC_THICKNESS / FAR_PLANE;
+        float o = 0.0f;
+        if (g_clamp(su, 0.0f, 1.0f) == su && g_clamp(sv, 0.0f, 1.0f) == sv && mn < sd && sd < mx) {
+            /* smoothstep(maxDepth, minDepth, d) with its edges reversed, as written */
+            const float tt = g_clamp((sd - mx) / (mn - mx), 0.0f, 1.0f);
+            o = ((tt * tt) * (3.0f - 2.0f * tt)) * dist;
+        }
+        occ = occ + o;
+    }
+    occ = occ / 4.0f;
+    occ = occ * OCC_STRENGTH;
+    return g_clamp(1.0f - occ, 0.0f, 1.0f);
+}
+
+void vxo_ssao(const vxo_view* vw, const float* nrm, const uint8_t* mat, const float* depth, uint32_t seed,
+              float* out, int32_t nthreads) {
+    const int32_t npx = vw->width * vw->height;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (int32_t i = 0; i < npx; ++i)
+        out[i] = (mat[i] != 0 && depth[i] < 0.999f) ? ssao_pixel(vw, nrm, depth, seed, (size_t)i) : 1.0f;
     (void)nthreads;
 }
 

@@ -127,6 +127,11 @@ int32_t vxo_intersect_volume(const uint8_t* const mips[3], const int32_t dims[3]
 /* ---- full passes ---- */
 void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* view,
                  float* pos, float* nrm, uint8_t* mat, int32_t nthreads);
+/* vxo_primary for the listed pixels only; outputs are compact (entry j for
+ * pixel pixels[j]). */
+void vxo_primary_pixels(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* view,
+                        const int32_t* pixels, int32_t npixels, float* pos, float* nrm, uint8_t* mat,
+                        int32_t nthreads);
 /* One voxel draw (GeometryVoxelPipeline::Cmd, GeometryVoxelPipeline.h:29-35)
  * with its model's mips. */
 typedef struct {
@@ -208,6 +213,39 @@ typedef struct {
 void vxo_light_pass(const vxo_occ* v, const vxo_view* view, const vxo_light* lights, int32_t nlights,
                     const vxo_light_params* p, const float* pos, const float* nrm, const uint8_t* mat,
                     const float* palette, float* rgb, uint32_t* fetches, int32_t nthreads);
+
+/* ---- reflection pass (LightReflection.frag:60-139) ---- */
+typedef struct {
+    int32_t march;      /* reference: SPARSE (LightReflection.frag:113) */
+    float maxt;         /* reference: 256 */
+    uint32_t seed;
+    int32_t frame;
+} vxo_reflection_params;
+
+/* getSkyColor (LightAmbient.frag:128-131) of direction e. */
+void vxo_sky_procedural(const float e[3], float out[3]);
+/* Cube-map lookup: faces +X, -X, +Y, -Y, +Z, -Z of size x size RGBA. */
+void vxo_cube_sample(const float* faces, int32_t size, const float d[3], float out[4]);
+/* The reflection ray of each listed pixel: wcp (fine units) and direction,
+ * 6 floats; the march starts at wcp + normal. */
+void vxo_reflection_rays(const vxo_view* view, const vxo_reflection_params* p, const float* pos, const float* nrm,
+                         const uint8_t* mat, const float* palette, const int32_t* pixels, int32_t npixels,
+                         float* rays);
+/* out: 4 floats per pixel, (ambient * F * (1 - roughness), F). depth: the D24
+ * depth per pixel (1 = sky); light: 4 floats per pixel; sky: NULL for the
+ * procedural sky. t_out (optional): the marched distance, NaN where no ray. */
+void vxo_reflection(const vxo_occ* v, const vxo_view* view, const vxo_reflection_params* p, const float* pos,
+                    const float* nrm, const uint8_t* mat, const float* palette, const float* depth,
+                    const float* light, const float* sky, int32_t sky_size, float* out, float* t_out,
+                    int32_t nthreads);
+
+/* ---- screen-space occlusion (LightAmbient.frag:59-109) ---- */
+/* calculateOcclusion(N) per pixel with depth < 0.999 and a hit (1 elsewhere):
+ * 4 samples within 0.2 world units of the view-space position rebuilt from
+ * the depth attachment, occluded where the depth attachment at the sample's
+ * projection lies within 0.2 / FAR in front of it; 1 - 3 * mean, clamped. */
+void vxo_ssao(const vxo_view* view, const float* nrm, const uint8_t* mat, const float* depth, uint32_t seed,
+              float* out, int32_t nthreads);
 
 /* ---- synthetic scenes (inputs, not reference code) ---- */
 void vxo_gen_terrain(uint8_t* vol, int32_t nx, int32_t ny, int32_t nz, uint32_t seed,
