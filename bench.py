@@ -34,16 +34,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per k_march launch on this workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known
-    size), or None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_march_F_1080p_512_traffic.json")
+KERNEL_SRC = os.path.join(ROOT, "voxelengine_amd", "csrc", "vx_kernels.hip")
+COUNTERS = os.path.join(ROOT, "profiles", "k_march_F_1080p_512_counters.json")
+MI355X_SIMDS = 1024          # 256 CUs x 4 SIMDs
+MI355X_CLOCK_HZ = 2.4e9      # peak engine clock
+
+
+def kernel_src_sha16():
+    import hashlib
+    with open(KERNEL_SRC, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def profiled_counters():
+    """rocprofv3 counters of k_march on this workload (HBM traffic from the
+    FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel of known size, L2
+    hits and misses, vector / scalar instructions), summarized by
+    `bench.py --summarize-profiles` into the committed counters file. Returned
+    only when they were measured on the kernel source of this tree (its
+    sha256 prefix is stored with them); otherwise None."""
     try:
-        with open(path) as f:
-            return json.load(f)["traffic_bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
+        with open(COUNTERS) as f:
+            c = json.load(f)
+    except (OSError, ValueError):
         return None
+    return c if c.get("kernel_src_sha16") == kernel_src_sha16() else None
 
 
 def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
@@ -69,7 +84,37 @@ def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
     rays = reps * sample.size * (params["n_shadow"] + params["n_ao"] + params["n_spec"])
     what = "the whole frame" if stride == 1 else f"every {stride}th hit pixel"
     return {"value": rays / t_total / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
+            "label": "C++ restatement (not lavapipe): the oracle, C with OpenMP over pixels",
+            "cpu_model": cpu_model(),
             "sample": f"{what} ({sample.size} px) x {reps}: {rays} rays in {t_total:.1f} s"}
+
+
+def limiter(counters, kernel_ms):
+    """What the rocprofv3 counters of this kernel source show bounds k_march:
+    measured HBM traffic against the HBM roofline, and vector-instruction issue
+    (SQ_INSTS_VALU wave instructions x 4 cycles per wave64 instruction over
+    1024 SIMDs at 2.4 GHz) against the kernel time; None without counters."""
+    if not counters:
+        return None
+    t = kernel_ms * 1e-3
+    hbm = counters["traffic_bytes_per_launch"] / t / (HBM_PEAK_GBS * 1e9)
+    issue_s = counters["sq_insts_valu_per_launch"] * 4.0 / (MI355X_SIMDS * MI355X_CLOCK_HZ)
+    return {"bound": "valu_issue" if issue_s / t > hbm else "hbm",
+            "valu_issue_ms": round(issue_s * 1e3, 4), "valu_issue_frac": round(issue_s / t, 4),
+            "hbm_traffic_frac": round(hbm, 4), "l2_hit_rate": counters["l2_hit_rate"],
+            "lane_use": counters.get("lane_use"),
+            "counters": os.path.relpath(COUNTERS, ROOT) + " @ kernel source " + counters["kernel_src_sha16"]}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def bench_batched(ctx, pal_d, reps=5):
@@ -77,7 +122,7 @@ def bench_batched(ctx, pal_d, reps=5):
     each (256x256..1920x1080), 1 shadow + 8 AO rays: one graph-captured
     persistent launch against 64 plain per-view launches."""
     import torch
-    from voxelengine_amd import api, configs, views
+    from voxelengine_amd import api, autotune, configs, views
     params = configs.trace_params(n_shadow=1, n_ao=8, ao_maxt=32.0, n_spec=0, march="dda")
     items = []
     for sc in configs.batched_scenes():
@@ -98,15 +143,23 @@ def bench_batched(ctx, pal_d, reps=5):
     hit = sum(int((it[3].material > 0).sum().item()) for it in items)
     rays = hit * configs.rays_per_pixel(params)
     bvis = [api.Visibility(it[2]["width"], it[2]["height"]) for it in items]
-    batch = api.Batch(ctx, [(it[1], it[2], it[3], bv) for it, bv in zip(items, bvis)], pal_d, params)
 
     def plain():
         for (m, occ, vd, g, vis) in items:
             api.trace(ctx, occ, vd, g, pal_d, params, vis)
 
+    # the autotuner picks the launch shape of each side on this workload:
+    # one tuning for the 64 plain launches, one for the batched graph
+    cache = os.path.join(ROOT, "gpurun_out", "tune_cache.json")
+    tuned_plain, plain_ms = autotune.tune(ctx, plain, "bench-B-plain", cache_path=cache, reps=2, force=True)
+    batch, tuned_batch, batch_ms = autotune.tune_batch(
+        ctx, lambda: api.Batch(ctx, [(it[1], it[2], it[3], bv) for it, bv in zip(items, bvis)], pal_d, params),
+        "bench-B-batched", cache_path=cache, reps=2, force=True)
+
     res = {}
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for name, fn in (("plain_64_launches", plain), ("batched_graph", batch.launch)):
+    for name, fn, tu in (("plain_64_launches", plain, tuned_plain), ("batched_graph", batch.launch, tuned_batch)):
+        ctx.set_tuning(**tu)
         fn()
         torch.cuda.synchronize()
         e0.record()
@@ -115,23 +168,25 @@ def bench_batched(ctx, pal_d, reps=5):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
-        res[name] = {"ms": round(ms, 3), "grays_per_s": round(rays / (ms * 1e-3) / 1e9, 4)}
+        res[name] = {"ms": round(ms, 3), "grays_per_s": round(rays / (ms * 1e-3) / 1e9, 4), "tuning": tu}
     same = all(torch.equal(a[4].ao.view(torch.int32), b.ao.view(torch.int32)) and torch.equal(a[4].shadow, b.shadow)
                for a, b in zip(items, bvis))
     res["rays"] = rays
     res["batched_vs_plain"] = round(res["plain_64_launches"]["ms"] / res["batched_graph"]["ms"], 3)
     res["identical_outputs"] = bool(same)
+    res["autotune_ms"] = {"plain": plain_ms, "batched": batch_ms}
     batch.close()
+    ctx.reset_tuning()
     return res
 
 
 def bench_large(ctx, pal_d, reps=3):
-    """Config L: 1024^3 city blocks (20000 grounded boxes), 3 levels, four
-    3840x2160 views with 1 shadow + 32 AO + 1 spec ray per hit pixel in one
-    graph-captured launch; palette and light gradients of all four views
-    accumulated in one adjoint launch."""
+    """Config L: 1024^3 city blocks (20000 grounded boxes of edges 8..56, 5.2 %
+    fill), 3 levels, four 3840x2160 views from inside the city with 1 shadow +
+    32 AO + 1 spec ray per hit pixel in one graph-captured launch; palette and
+    light gradients of all four views accumulated in one adjoint launch."""
     import torch
-    from voxelengine_amd import api, configs, views
+    from voxelengine_amd import api, autotune, configs, views
     cfg = configs.CONFIGS["L"]
     nx, ny, nz = cfg["dims"]
     W, H = cfg["width"], cfg["height"]
@@ -146,14 +201,17 @@ def bench_large(ctx, pal_d, reps=3):
     occ.build_levels()
     items = []
     for k in range(cfg["nviews"]):
-        vd = views.scene_view(cfg["dims"], W, H, variant=k)
+        vd = configs.config_view(cfg, variant=k)
         g = api.GBuffer(W, H)
         api.primary(ctx, m, vd, g)
         items.append((occ, vd, g, api.Visibility(W, H)))
     torch.cuda.synchronize()
     hit = sum(int((it[2].material > 0).sum().item()) for it in items)
     rays = hit * configs.rays_per_pixel(params)
-    batch = api.Batch(ctx, items, pal_d, params)
+    batch, tuned, tune_ms = autotune.tune_batch(
+        ctx, lambda: api.Batch(ctx, items, pal_d, params), "bench-L", reps=1, force=True,
+        cache_path=os.path.join(ROOT, "gpurun_out", "tune_cache.json"),
+        candidates=[c for c in autotune.MARCH_CANDIDATES if c["refill"] == 16 and c["steps"] == 8])
     batch.launch()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -177,7 +235,9 @@ def bench_large(ctx, pal_d, reps=3):
     torch.cuda.synchronize()
     bwd_ms = e0.elapsed_time(e1) / 5
     batch.close()
-    out = {"views": cfg["nviews"], "resolution": f"{W}x{H}", "hit_pixels": hit, "rays": rays,
+    ctx.reset_tuning()
+    out = {"views": cfg["nviews"], "resolution": f"{W}x{H}", "hit_pixels": hit, "tuning": tuned,
+           "hit_fraction": round(hit / (cfg["nviews"] * W * H), 4), "rays": rays,
            "forward_ms": round(fwd_ms, 3), "forward_grays_per_s": round(rays / (fwd_ms * 1e-3) / 1e9, 3),
            "backward_ms": round(bwd_ms, 4), "backward_grays_per_s": round(rays / (bwd_ms * 1e-3) / 1e9, 2),
            "backward_px_per_s": round(hit / (bwd_ms * 1e-3), 1)}
@@ -281,20 +341,30 @@ def bench_lights(ctx, occ, vd, g, pal_d, dims, reps=10):
     api.light_pass(ctx, occ, vd, g, pal_d, lights, L.size, rgb, seed=3, frame=1, fetches=fetches)
     torch.cuda.synchronize()
     s_ref = int(fetches.to(torch.int64).sum().item())
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        api.light_pass(ctx, occ, vd, g, pal_d, lights, L.size, rgb, seed=3, frame=1)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
     # algorithmic bytes: one byte per reference fetch, plus per hit pixel the
     # G-buffer read (25 B) and the rgb read and write (24 B)
     alg = s_ref + 49 * int(hit.sum().item())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = {}
+    for name, path in (("ray_list", 0), ("pixel_loop", 1)):
+        ctx.set_tuning(lights=path)
+        api.light_pass(ctx, occ, vd, g, pal_d, lights, L.size, rgb, seed=3, frame=1)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            api.light_pass(ctx, occ, vd, g, pal_d, lights, L.size, rgb, seed=3, frame=1)
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / reps
+    ctx.reset_tuning()
+    ms = times["ray_list"]
     return {"point_lights": 64, "spot_lights": 64, "marchers": "sparse (point) / supersparse (spot)",
+            "kernel": "ray list (count, list, march one ray per lane, fold)",
             "ms": round(ms, 4), "shadow_rays": rays, "grays_per_s": round(rays / (ms * 1e-3) / 1e9, 3),
             "ref_fetches": s_ref, "achieved_gbps": round(alg / (ms * 1e-3) / 1e9, 1),
-            "frac_of_hbm": round(alg / (ms * 1e-3) / 8.0e12, 4)}
+            "frac_of_hbm": round(alg / (ms * 1e-3) / 8.0e12, 4),
+            "pixel_loop_ms": round(times["pixel_loop"], 4),
+            "pixel_loop_grays_per_s": round(rays / (times["pixel_loop"] * 1e-3) / 1e9, 3)}
 
 
 def bench_prefabs(ctx, reps=5):
@@ -344,7 +414,67 @@ def bench_prefabs(ctx, reps=5):
     return res
 
 
+def _counter_rows(path):
+    """{dispatch id: (kernel name, {counter: value})} of a rocprofv3
+    run_counter_collection.csv."""
+    import csv
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = out.setdefault(int(r["Dispatch_Id"]), (r["Kernel_Name"], {}))
+            d[1][r["Counter_Name"]] = d[1].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def summarize_profiles(d):
+    """Per-launch k_march counters from rocprofv3 passes over
+    `bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-extras --copy-probe`
+    under d: pmc_fetch (FETCH_SIZE), pmc_write (WRITE_SIZE), pmc_tcc
+    (TCC_HIT_sum TCC_MISS_sum), pmc_sq (SQ_INSTS_VALU SQ_INSTS_SALU
+    SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_WAIT_INST_ANY SQ_WAVE_CYCLES).
+    The timed launches (k_march without fetch counts, the first of them
+    excluded: cold caches) are averaged. FETCH_SIZE / WRITE_SIZE are scaled by
+    what they read for the 1 GiB device copy of --copy-probe. Writes the
+    counters file and returns it."""
+    march = "k_march<256, 8, true, false>"
+
+    def per_launch(sub, names):
+        rows = _counter_rows(os.path.join(d, sub, "run_counter_collection.csv"))
+        ks = [v for _, (k, v) in sorted(rows.items()) if march in k][1:]
+        copies = [v for _, (k, v) in sorted(rows.items()) if "copyBuffer" in k]
+        avg = {n: sum(v[n] for v in ks) / len(ks) for n in names}
+        return avg, copies, len(ks)
+
+    f, fc, n1 = per_launch("pmc_fetch", ["FETCH_SIZE"])
+    w, wc, n2 = per_launch("pmc_write", ["WRITE_SIZE"])
+    t, _, n3 = per_launch("pmc_tcc", ["TCC_HIT_sum", "TCC_MISS_sum"])
+    q, _, n4 = per_launch("pmc_sq", ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU",
+                                     "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"])
+    gib = float(1 << 30)
+    fscale = gib / (max(c["FETCH_SIZE"] for c in fc) * 1024.0)
+    wscale = gib / (max(c["WRITE_SIZE"] for c in wc) * 1024.0)
+    fetch_b = f["FETCH_SIZE"] * 1024.0 * fscale
+    write_b = w["WRITE_SIZE"] * 1024.0 * wscale
+    c = {"kernel": march, "workload": "F: 512^3, 1920x1080, 1 shadow + 16 AO(32) + 1 spec DDA rays per hit pixel",
+         "kernel_src_sha16": kernel_src_sha16(), "launches_averaged": [n1, n2, n3, n4],
+         "fetch_size_scale": round(fscale, 4), "write_size_scale": round(wscale, 4),
+         "hbm_fetch_bytes_per_launch": int(fetch_b), "hbm_write_bytes_per_launch": int(write_b),
+         "traffic_bytes_per_launch": int(fetch_b + write_b),
+         "tcc_hit_per_launch": int(t["TCC_HIT_sum"]), "tcc_miss_per_launch": int(t["TCC_MISS_sum"]),
+         "l2_hit_rate": round(t["TCC_HIT_sum"] / (t["TCC_HIT_sum"] + t["TCC_MISS_sum"]), 4),
+         "sq_insts_valu_per_launch": int(q["SQ_INSTS_VALU"]), "sq_insts_salu_per_launch": int(q["SQ_INSTS_SALU"]),
+         "lane_use": round(q["SQ_THREAD_CYCLES_VALU"] / (64.0 * q["SQ_ACTIVE_INST_VALU"]), 4),
+         "wait_frac": round(q["SQ_WAIT_INST_ANY"] / q["SQ_WAVE_CYCLES"], 4)}
+    with open(COUNTERS, "w") as fo:
+        json.dump(c, fo, indent=1)
+        fo.write("\n")
+    return c
+
+
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--summarize-profiles":
+        print(json.dumps(summarize_profiles(sys.argv[2]), indent=1))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -408,6 +538,14 @@ def main():
     torch.cuda.synchronize()
     s_ref = int(vis.fetches.to(torch.int64).sum().item())
     alg_bytes = s_ref + PIXEL_IO_BYTES * hit_px
+    # fetches the kernel executes: a ray that ends by the step limit or maxT is
+    # marched once, where the reference repeats its pass four times
+    # (Light.frag:79); from every ray's hit record (term, steps, fetches)
+    hits = api.trace_hits_device(ctx, occ, vd, g, pal_d, params, api.Visibility(W, H))
+    term, steps, fetches = hits[:, 0].to(torch.int64), hits[:, 1].to(torch.int64), hits[:, 2].to(torch.int64)
+    s_exec = int(torch.where(term == 0, steps // 4, fetches).sum().item())
+    del hits, term, steps, fetches
+    torch.cuda.empty_cache()
     vis_t = api.Visibility(W, H, fetches=False)
 
     # ---- timed region: K trace launches on the current stream; the library
@@ -431,7 +569,8 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = rays_per_frame / (ms_per_step * 1e-3) / 1e9
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    traffic = measured_traffic()
+    exec_bytes = s_exec + PIXEL_IO_BYTES * hit_px
+    counters = profiled_counters()
     if args.copy_probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -457,9 +596,13 @@ def main():
                                "1 GGX spec-occ ray per hit pixel, DDA marcher",
                    "hit_pixels": hit_px, "rays_per_frame": rays_per_frame, "marcher": "dda"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": counters["traffic_bytes_per_launch"] if counters else None,
                      "kernel": "k_march", "kernel_ms": round(kernel_ms, 4), "trace_ms": round(trace_ms, 4),
-                     "alg_bytes_per_launch": alg_bytes, "ref_fetches_per_launch": s_ref},
+                     "alg_bytes_per_launch": alg_bytes, "ref_fetches_per_launch": s_ref,
+                     "executed_fetches_per_launch": s_exec,
+                     "frac_executed": round(exec_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "limiter": limiter(counters, kernel_ms)},
         "rebuild_ms": round(rebuild_ms, 3),
         "primary_ms": round(primary_ms, 4),
     }

@@ -302,18 +302,43 @@ public:
         p.n_spec = 0;
         check(vx_trace(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, &p, &lightOut, cmd));
     }
+    // calculateOcclusion (LightAmbient.frag:89-109) from the depth attachment:
+    // the screen-space factor of the ambient term, for vx_shade_forward.
+    void UseOcclusion(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB,
+                      const float* geometryDepth, uint32_t seed, float* occlusionOut) {
+        check(vx_ssao(ctx.get(), &viewBuffer, &geometryFB, geometryDepth, seed, occlusionOut, cmd));
+    }
 };
 
-// LightReflectionPipeline (LightReflectionPipeline.h:12-57): the specular
-// occlusion ray.
+// LightReflectionPipeline (LightReflectionPipeline.h:12-57):
+// Use(cmd, viewBuffer, lightDiffuse, geometryFB, shadowVox, skybox, blueNoise)
+// draws the reflection buffer (LightReflection.frag:60-139): per pixel, the
+// reflection ray marched through the ShadowVox volume, reprojected into
+// lightDiffuse (the TAA light buffer, 4 floats per pixel) when it lands within
+// the depth window, the skybox on a miss, out = (ambient * F * (1 -
+// roughness), F). The geometry framebuffer is the G-buffer plus its depth
+// attachment; the blue-noise texture is the seeded counter hash of params.
 class LightReflectionPipeline {
 public:
     static LightReflectionPipeline& Get() {
         static LightReflectionPipeline p;
         return p;
     }
-    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB, vx_volume* shadowVox,
-             const float* palette, const vx_trace_params& params, const vx_trace_out& reflectionOut) {
+    void Use(Context& ctx, void* cmd, const vx_view& viewBuffer, const float* lightDiffuse, const vx_gbuf& geometryFB,
+             const float* geometryDepth, vx_volume* shadowVox, const vx_skybox& skybox, const float* palette,
+             const vx_reflection_params& params, float* reflectionOut) {
+        vx_reflection_io io;
+        io.light = lightDiffuse;
+        io.depth = geometryDepth;
+        io.out = reflectionOut;
+        io.t = nullptr;
+        check(vx_reflection(ctx.get(), shadowVox, &viewBuffer, &geometryFB, palette, &io, &skybox, &params, cmd));
+    }
+    // The build's GGX specular-occlusion ray alone (spec = t / maxT per
+    // pixel), the ray class vx_trace marches beside shadow and AO.
+    void UseSpecularOcclusion(Context& ctx, void* cmd, const vx_view& viewBuffer, const vx_gbuf& geometryFB,
+                              vx_volume* shadowVox, const float* palette, const vx_trace_params& params,
+                              const vx_trace_out& reflectionOut) {
         vx_trace_params p = params;
         p.n_shadow = 0;
         p.n_ao = 0;

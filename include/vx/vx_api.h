@@ -136,6 +136,18 @@ typedef struct {
     float normal[3];   /* DDA: -sign(dir)*select */
 } vx_ray_result;
 
+/* One ray of the trace as raycastShadowVolume reports it (Light.frag:57-63,
+ * 73-79), written by vx_trace_hits; 40 bytes. Fields other than term, steps
+ * and fetches are 0 unless the ray hit (term 1). */
+typedef struct {
+    int32_t term;      /* 0 loop exhausted (n == 256 or totalt >= maxT), 1 hit, 2 out of bounds */
+    int32_t steps;     /* the reference's step count nt (the x4 restart of a term-0 ray included) */
+    int32_t fetches;   /* occupancy-byte fetches of the reference algorithm */
+    int32_t voxel[3];  /* hit voxel (fine units) */
+    float t;           /* best_t = dot(t_max, select) at the hit */
+    float normal[3];   /* -sign(dir) * select at the hit: the face the ray leaves the voxel by */
+} vx_trace_hit;
+
 /* Synthetic scene description (build-defined inputs for tests and benchmarks). */
 typedef struct {
     uint32_t seed;
@@ -154,7 +166,9 @@ typedef struct {
     int32_t blocks_per_cu;   /* persistent grid = CUs x this (0 = occupancy limit) */
     int32_t refill;          /* lanes without a running ray that trigger a refill, 1..64 (0 = 16) */
     int32_t steps;           /* DDA steps between refill checks: 2, 4 or 8 (0 = 8) */
-    int32_t pad[3];
+    int32_t tile;            /* pixels of a ray-parallel tile: 0 = 8x8, 1 = 16x4, 2 = 4x16 */
+    int32_t order;           /* tile queue order: 0 = rows of tiles, 1 = rows of 8x8-tile blocks */
+    int32_t lights;          /* vx_light_pass: 0 = list of (pixel, light) rays, one per lane; 1 = a loop per pixel */
 } vx_trace_tuning;
 
 /* ---------------------------------------------------------------- context */
@@ -264,6 +278,14 @@ vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, 
 vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
                    const float* palette_dev, const vx_trace_params* params, const vx_trace_out* out,
                    void* stream);
+/* vx_trace (all rays through the DDA marcher) that also writes every ray's
+ * hit record: hits_dev[pixel * nrays + r] for ray r of a pixel (r in the
+ * order shadow, AO, spec; nrays = n_shadow + n_ao + n_spec), all zero for sky
+ * pixels. The same ray-parallel kernel as vx_trace runs, in its step-counting
+ * form. VX_ERR_UNSUPPORTED unless every ray class uses VX_MARCH_DDA. */
+vx_status vx_trace_hits(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
+                        const float* palette_dev, const vx_trace_params* params, const vx_trace_out* out,
+                        vx_trace_hit* hits_dev, void* stream);
 /* Explicit rays (device, n*7 floats: origin3, dir3, maxt) through one marcher. */
 vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const float* rays_dev, uint32_t n,
                         int32_t march, vx_ray_result* out_dev, void* stream);
@@ -357,6 +379,73 @@ typedef struct {
  * last position left the screen average a 19x19 same-surface neighbourhood.
  * Uses view width/height and frame; seed keys the per-pixel spiral angle. */
 vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa_io* io, uint32_t seed, void* stream);
+
+/* ---------------------------------------------------------------- reflection pass
+ * LightReflectionPipeline::Use(cmd, viewBuffer, lightDiffuse, geometryFB,
+ * shadowVox, skybox, blueNoise) (LightReflectionPipeline.h:34) with
+ * LightReflection.frag:60-139. Per pixel whose depth is below 0.999: the
+ * reflection ray (reflect(-V, N) mixed with a cosine sample by roughness *
+ * 0.1 and scaled by 1 + noise.w * 0.5, from the hit + normal * noise.w) is
+ * marched to maxt; on a miss (t == maxt) the sky is taken in the ray's
+ * direction, otherwise the hit is reprojected and, when its linear depth lies
+ * within +-0.001 of the depth attachment there, the light buffer is taken
+ * there. Output (ambient * F * (1 - roughness), F) with F =
+ * fresnelSchlickRoughness(max(N.V, 0), mix(0.04, 1, metallic), roughness) * 5
+ * (albedo 1, LightReflection.frag:74-86). The light and depth images are
+ * sampled bilinearly with 8-bit sub-texel weights and an opaque-black border.
+ * Noise: the pixel's seeded counter hash (the blue-noise texel's stand-in). */
+typedef struct {
+    int32_t march;      /* reference: VX_MARCH_SPARSE (LightReflection.frag:113) */
+    float maxt;         /* reference: 256 */
+    uint32_t seed;
+    int32_t frame;
+} vx_reflection_params;
+
+/* The skybox (SkyBoxAsset's cube image): faces +X, -X, +Y, -Y, +Z, -Z (Vulkan
+ * layer order), each size x size RGBA fp32 texels, rows from t = 0 (device).
+ * A lookup selects the face of the direction's major axis (x, then y, then z
+ * on ties) and filters bilinearly within the face, the texel position
+ * clamped to it. faces NULL: the procedural getSkyColor of
+ * LightAmbient.frag:128-131, which LightReflection.frag:55-56 keeps as a
+ * comment beside its skybox lookup. */
+typedef struct {
+    const float* faces;
+    int32_t size;
+    int32_t pad;
+} vx_skybox;
+
+typedef struct {
+    const float* light;   /* lightDiffuse: 4 floats per pixel (the TAA light buffer), 16-byte aligned */
+    const float* depth;   /* the depth attachment as read back (vx_gbuf_ext.depth or vx_gbuf_depth), 1 = sky */
+    float* out;           /* 4 floats per pixel (the reflection buffer), 16-byte aligned */
+    float* t;             /* optional: the ray's marched distance per pixel, NaN where no ray was marched */
+} vx_reflection_io;
+
+vx_status vx_reflection(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
+                        const float* palette_dev, const vx_reflection_io* io, const vx_skybox* sky,
+                        const vx_reflection_params* params, void* stream);
+
+/* The depth attachment of a G-buffer written by vx_primary (which writes no
+ * depth): per pixel the geometry pass's fragment depth (1 - 1e-7 volume_rid)
+ * (w - NEAR) / (FAR - NEAR) of the hit (GeometryVoxel.frag:167-170), quantized
+ * to D24 and read back as q / (2^24 - 1), 1 where nothing was hit; the same
+ * values vx_primary_instances writes to vx_gbuf_ext.depth. depth_dev: npx. */
+vx_status vx_gbuf_depth(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbuf, int32_t volume_rid, float* depth_dev,
+                        void* stream);
+
+/* Screen-space occlusion of the ambient pass: calculateOcclusion(N)
+ * (LightAmbient.frag:89-109, with screenspaceOcclusion :59-79), the factor of
+ * the ambient term that vx_shade_forward takes as ssao_dev. Per pixel with a
+ * hit and depth below 0.999: the view-space position rebuilt from the depth
+ * attachment (FarVec * (depth * (1 + 1/FAR) + NEAR/FAR)), four samples within
+ * 0.2 world units in the cosine hemisphere of the view-space normal, each
+ * occluded where the depth attachment at its projection lies within
+ * 0.2 / FAR in front of it (weight smoothstep(max, min, depth) * distance);
+ * ssao = clamp(1 - 3 * mean, 0, 1); 1 elsewhere. Noise: the pixel's seeded
+ * counter hash (the blue-noise texel's stand-in). depth_dev: the depth
+ * attachment (vx_gbuf_ext.depth or vx_gbuf_depth); ssao_dev: npx floats. */
+vx_status vx_ssao(vx_ctx* ctx, const vx_view* view, const vx_gbuf* gbuf, const float* depth_dev, uint32_t seed,
+                  float* ssao_dev, void* stream);
 
 /* One view of a batched adjoint: its G-buffer, fixed visibility, optional
  * SSAO and the loss gradient w.r.t. its shaded image (all device). */

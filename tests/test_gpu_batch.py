@@ -86,3 +86,49 @@ def test_batch_rejects_reference_marchers(ctx):
     from voxelengine_amd import views
     with pytest.raises(_lib.VXError):
         api.Batch(ctx, [(vol, views.scene_view((16, 16, 16), 8, 8), g, vis)], pal, params)
+
+
+def test_every_autotuner_candidate_gives_the_same_bits(ctx, tmp_path):
+    """Each tile shape, queue order and refill setting the autotuner tries,
+    for a batch and for the plain trace, equals the default launch bit for
+    bit; tune_batch returns a working batch under the winning tuning."""
+    from voxelengine_amd import api, autotune, configs, views
+    items = _items(api, ctx, views, configs, n=5, seed=9)
+    pal = torch.from_numpy(configs.make_palette()).cuda()
+    params = configs.trace_params(n_ao=5, ao_maxt=24.0, march="dda")
+    ref = []
+    for it in items:
+        vis = api.Visibility(it["w"], it["h"], fetches=True)
+        api.trace(ctx, it["occ"], it["vd"], it["g"], pal, params, vis)
+        ref.append(vis)
+    try:
+        for cand in autotune.CANDIDATES:
+            ctx.set_tuning(**cand)
+            outs = [api.Visibility(it["w"], it["h"], fetches=True) for it in items]
+            for it, o in zip(items, outs):
+                api.trace(ctx, it["occ"], it["vd"], it["g"], pal, params, o)
+            if cand.get("kernel") == 2:
+                bouts = [api.Visibility(it["w"], it["h"], fetches=True) for it in items]
+                b = api.Batch(ctx, [(it["occ"], it["vd"], it["g"], o) for it, o in zip(items, bouts)], pal, params)
+                b.launch()
+                torch.cuda.synchronize()
+                b.close()
+                outs = outs + bouts
+            torch.cuda.synchronize()
+            for k, o in enumerate(outs):
+                p = ref[k % len(items)]
+                for f in ("shadow", "ao", "spec", "fetches"):
+                    assert_bitwise(getattr(o, f).cpu().numpy(), getattr(p, f).cpu().numpy(), f"{cand} item {k} {f}")
+        ctx.reset_tuning()
+        outs = [api.Visibility(it["w"], it["h"]) for it in items]
+        batch, best, times = autotune.tune_batch(
+            ctx, lambda: api.Batch(ctx, [(it["occ"], it["vd"], it["g"], o) for it, o in zip(items, outs)], pal, params),
+            "test-batch", cache_path=str(tmp_path / "tune.json"), reps=1)
+        assert len(times) == len(autotune.MARCH_CANDIDATES) and best in autotune.MARCH_CANDIDATES
+        batch.launch()
+        torch.cuda.synchronize()
+        for k, o in enumerate(outs):
+            assert_bitwise(o.ao.cpu().numpy(), ref[k].ao.cpu().numpy(), f"tuned batch item {k}")
+        batch.close()
+    finally:
+        ctx.reset_tuning()

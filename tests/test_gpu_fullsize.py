@@ -74,3 +74,30 @@ def test_fullsize_trace_every_pixel(frame_f, oracle, ctx, march):
     assert_bitwise(vis2.ao.cpu().numpy(), ao, "ao rerun")
     assert_bitwise(vis2.shadow.cpu().numpy(), sh, "shadow rerun")
     assert_bitwise(vis2.spec.cpu().numpy(), sp, "spec rerun")
+
+
+def test_fullsize_hit_records_every_ray(frame_f, oracle, ctx):
+    """Hit voxel, hit face (normal), best_t, step count and fetch count of all
+    27.1M rays of the frame from the headline kernel (vx_trace_hits), against
+    the oracle's raycastShadowVolume on the same rays (Light.frag:29-81)."""
+    from voxelengine_amd import api, configs
+    cfg = frame_f["cfg"]
+    params = configs.trace_params(**cfg["params"])
+    g, vd = frame_f["g"], frame_f["vd"]
+    W, H = cfg["width"], cfg["height"]
+    pal = configs.make_palette()
+    got = api.trace_hits(ctx, frame_f["occ"], vd, g, torch.from_numpy(pal).cuda(), params, api.Visibility(W, H))
+    occ_h = frame_f.get("occ_h")
+    if occ_h is None:
+        occ_h = frame_f["occ"].download()
+    pos, nrm, mat = g.pos.cpu().numpy(), g.normal.cpu().numpy(), g.material.cpu().numpy()
+    hit_px = np.nonzero(mat)[0].astype(np.int32)
+    assert not got[mat == 0].view(np.uint8).any()
+    got = got[hit_px]
+    rays = oracle.pixel_rays(vd, params, pos, nrm, mat, pal, hit_px)
+    ref = oracle.trace_rays(occ_h, rays.reshape(-1, 7), 0).reshape(hit_px.size, -1)
+    del rays
+    for f in ("term", "steps", "fetches", "voxel", "t", "normal"):
+        assert_bitwise(got[f], ref[f], f"F frame {f}")
+    terms, counts = np.unique(ref["term"], return_counts=True)
+    assert set(terms.tolist()) == {0, 1, 2} and counts.sum() == 27093780

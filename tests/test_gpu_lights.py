@@ -1,6 +1,8 @@
 """Point/spot light pass on the GPU against the oracle, bit for bit: rgb
 accumulated over lights in order and the marchers' fetch counts, with the
-reference marchers (Sparse for point, SuperSparse for spot) and with the DDA."""
+reference marchers (Sparse for point, SuperSparse for spot) and with the DDA,
+through both kernels of the pass (the (pixel, light) ray list, one ray per
+lane; and the per-pixel light loop)."""
 import numpy as np
 import pytest
 import torch
@@ -28,8 +30,15 @@ def _lights(dims, n_point=8, n_spot=5, seed=4):
     return L
 
 
+@pytest.fixture(params=[0, 1], ids=["ray_list", "pixel_loop"])
+def lights_path(request, ctx):
+    ctx.set_tuning(lights=request.param)
+    yield request.param
+    ctx.reset_tuning()
+
+
 @pytest.mark.parametrize("marchers", [(1, 2), (0, 0)])
-def test_light_pass_matches_oracle(ctx, oracle, marchers):
+def test_light_pass_matches_oracle(ctx, oracle, marchers, lights_path):
     from voxelengine_amd import api, configs, views
     model, occ, vol = device_scene(api, ctx, 64, seed=31, fill=0.3, nboxes=20)
     occ_h = oracle_occ(oracle, vol)
@@ -53,7 +62,7 @@ def test_light_pass_matches_oracle(ctx, oracle, marchers):
     assert lit.mean() > 0.2 and (~lit & (g.material.cpu().numpy() > 0)).any()
 
 
-def test_many_small_lights_match_oracle(ctx, oracle):
+def test_many_small_lights_match_oracle(ctx, oracle, lights_path):
     """150 lights with small ranges, so each 16x16 tile keeps only a few of them
     (the per-tile cull), plus ranges that are infinite, negative and zero:
     rgb and fetch counts bit for bit against the oracle, which tests every
@@ -83,3 +92,27 @@ def test_many_small_lights_match_oracle(ctx, oracle):
     assert_bitwise(fetches.cpu().numpy().astype(np.uint32), ref_f, "fetches")
     lit = (ref != base).any(axis=1)
     assert lit.mean() > 0.02
+
+
+def test_no_light_in_range_and_no_lights(ctx, oracle, lights_path):
+    """Lights whose ranges reach no hit pixel leave the light buffer as it was
+    (the ray list is empty), and so does a pass with no lights."""
+    from voxelengine_amd import api, configs, views
+    model, occ, vol = device_scene(api, ctx, 32, seed=33, fill=0.3, nboxes=5)
+    w, h = 40, 30
+    vd = views.scene_view(model.dims, w, h)
+    g = api.GBuffer(w, h)
+    api.primary(ctx, model, vd, g)
+    pal = torch.from_numpy(configs.make_palette()).cuda()
+    L = _lights(model.dims, n_point=3, n_spot=2)
+    L["position"] = [1000.0, 1000.0, 1000.0]
+    L["range"] = 1.0
+    base = np.random.default_rng(2).random((w * h, 3)).astype(np.float32)
+    for n in (L.size, 0):
+        rgb = torch.from_numpy(base.copy()).cuda()
+        fetches = torch.full((w * h,), 7, dtype=torch.int32, device="cuda")
+        api.light_pass(ctx, occ, vd, g, pal, api.lights_to_device(L), n, rgb, fetches=fetches)
+        torch.cuda.synchronize()
+        assert_bitwise(rgb.cpu().numpy(), base, f"rgb with {n} lights")
+        hit = g.material.cpu().numpy() > 0
+        assert (fetches.cpu().numpy()[hit] == 0).all()

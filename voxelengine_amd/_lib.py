@@ -91,7 +91,7 @@ class ShadeItem(C.Structure):
 
 class TraceTuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
-                ("steps", C.c_int32), ("pad", C.c_int32 * 3)]
+                ("steps", C.c_int32), ("tile", C.c_int32), ("order", C.c_int32), ("lights", C.c_int32)]
 
 
 class VoxCmd(C.Structure):
@@ -121,6 +121,18 @@ class PrefabEntity(C.Structure):
 ENT_TRANSFORM, ENT_VOX, ENT_LIGHT, ENT_INSTANCE = 1, 2, 4, 8
 
 # Every symbol the header declares, with its signature.
+class ReflectionParams(C.Structure):
+    _fields_ = [("march", C.c_int32), ("maxt", C.c_float), ("seed", C.c_uint32), ("frame", C.c_int32)]
+
+
+class SkyBox(C.Structure):
+    _fields_ = [("faces", C.c_void_p), ("size", C.c_int32), ("pad", C.c_int32)]
+
+
+class ReflectionIO(C.Structure):
+    _fields_ = [("light", C.c_void_p), ("depth", C.c_void_p), ("out", C.c_void_p), ("t", C.c_void_p)]
+
+
 SIGNATURES = {
     "vx_api_version": ([], C.c_int32),
     "vx_last_error": ([], C.c_char_p),
@@ -151,6 +163,8 @@ SIGNATURES = {
     "vx_primary": ([P, P, C.POINTER(View), C.POINTER(GBuf), P], C.c_int32),
     "vx_trace": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut), P],
                  C.c_int32),
+    "vx_trace_hits": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut),
+                       P, P], C.c_int32),
     "vx_trace_rays": ([P, P, P, C.c_uint32, C.c_int32, P, P], C.c_int32),
     "vx_batch_create": ([P, C.POINTER(BatchItem), C.c_uint32, P, C.POINTER(TraceParams), C.POINTER(P)], C.c_int32),
     "vx_batch_launch": ([P, P], C.c_int32),
@@ -179,6 +193,10 @@ SIGNATURES = {
     "vx_prefab_vox_cmds": ([P, C.POINTER(P), C.c_uint32, C.POINTER(C.c_float), C.POINTER(VoxCmd), C.c_uint32,
                             C.POINTER(C.c_uint32)], C.c_int32),
     "vx_light_taa": ([P, C.POINTER(View), C.POINTER(TaaIO), C.c_uint32, P], C.c_int32),
+    "vx_reflection": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(ReflectionIO), C.POINTER(SkyBox),
+                       C.POINTER(ReflectionParams), P], C.c_int32),
+    "vx_gbuf_depth": ([P, C.POINTER(View), C.POINTER(GBuf), C.c_int32, P, P], C.c_int32),
+    "vx_ssao": ([P, C.POINTER(View), C.POINTER(GBuf), P, C.c_uint32, P, P], C.c_int32),
     "vx_primary_instances": ([P, C.POINTER(VoxCmd), C.c_uint32, C.POINTER(View), C.POINTER(GBuf),
                               C.POINTER(GBufExt), P], C.c_int32),
 }

@@ -41,8 +41,8 @@ class Context:
         self.h = h
         self.device = device
 
-    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, steps=0):
-        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, steps)
+    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, steps=0, tile=0, order=0, lights=0):
+        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, steps, tile, order, lights)
         check(L.lib().vx_ctx_set_trace_tuning(self.h, C.byref(t)))
 
     def reset_tuning(self):
@@ -61,7 +61,8 @@ class Context:
     def tuning(self) -> dict:
         t = L.TraceTuning()
         check(L.lib().vx_ctx_get_trace_tuning(self.h, C.byref(t)))
-        return dict(kernel=t.kernel, block=t.block, blocks_per_cu=t.blocks_per_cu, refill=t.refill, steps=t.steps)
+        return dict(kernel=t.kernel, block=t.block, blocks_per_cu=t.blocks_per_cu, refill=t.refill, steps=t.steps,
+                    tile=t.tile, order=t.order, lights=t.lights)
 
     def close(self):
         if self.h:
@@ -303,6 +304,36 @@ def trace(ctx: Context, vol: Volume, vd: dict, gbuf: GBuffer, palette: torch.Ten
                            _stream(stream)))
 
 
+TRACE_HIT_DTYPE = np.dtype([("term", "<i4"), ("steps", "<i4"), ("fetches", "<i4"), ("voxel", "<i4", 3),
+                            ("t", "<f4"), ("normal", "<f4", 3)])
+
+
+def trace_hits_device(ctx: Context, vol: Volume, vd: dict, gbuf: GBuffer, palette: torch.Tensor, params: dict,
+                      vis: Visibility, stream=None) -> torch.Tensor:
+    """vx_trace_hits: the DDA trace plus every ray's hit record, as a device
+    int32 tensor of (width * height * rays per pixel, 10) words in
+    TRACE_HIT_DTYPE layout (term, steps, fetches, voxel xyz, t, normal xyz)."""
+    nr = params["n_shadow"] + params["n_ao"] + params["n_spec"]
+    npx = vd["width"] * vd["height"]
+    hits = torch.empty((npx * nr * TRACE_HIT_DTYPE.itemsize // 8,), dtype=torch.int64, device="cuda")
+    v = view_struct(vd)
+    g = gbuf.struct()
+    tp = trace_params(params)
+    o = vis.struct()
+    check(L.lib().vx_trace_hits(ctx.h, vol.h, C.byref(v), C.byref(g), _ptr(palette), C.byref(tp), C.byref(o),
+                                _ptr(hits), _stream(stream)))
+    return hits.view(torch.int32).view(npx * nr, TRACE_HIT_DTYPE.itemsize // 4)
+
+
+def trace_hits(ctx: Context, vol: Volume, vd: dict, gbuf: GBuffer, palette: torch.Tensor, params: dict,
+               vis: Visibility, stream=None) -> np.ndarray:
+    """vx_trace_hits with the records on the host, shaped (width * height,
+    rays per pixel)."""
+    nr = params["n_shadow"] + params["n_ao"] + params["n_spec"]
+    hits = trace_hits_device(ctx, vol, vd, gbuf, palette, params, vis, stream)
+    return hits.cpu().numpy().view(TRACE_HIT_DTYPE).reshape(-1, nr)
+
+
 RAY_RESULT_DTYPE = np.dtype([("hit", "<i4"), ("term", "<i4"), ("steps", "<i4"), ("fetches", "<i4"),
                              ("voxel", "<i4", 3), ("t", "<f4"), ("hitpos", "<f4", 3), ("normal", "<f4", 3)])
 
@@ -525,3 +556,33 @@ class Prefab:
             p = IDENTITY if prev is None else prev[i]
             out.append(make_instance(m.mip_ptr(0), m.dims, p, self.world(i), tuple(e.pivot)))
         return out
+
+
+def reflection(ctx: Context, vol: Volume, vd: dict, gbuf: GBuffer, palette: torch.Tensor, light: torch.Tensor,
+               depth: torch.Tensor, out: torch.Tensor, t: torch.Tensor = None, sky: torch.Tensor = None,
+               march: int = L.MARCH_SPARSE, maxt: float = 256.0, seed: int = 0, frame: int = 0, stream=None):
+    """LightReflection: out (npx x 4) = (ambient * F * (1 - roughness), F) from
+    the light buffer (npx x 4) and the depth attachment (npx); sky: a
+    (6, size, size, 4) cube map or None for the procedural sky."""
+    io = L.ReflectionIO(_ptr(light), _ptr(depth), _ptr(out), _ptr(t))
+    sb = L.SkyBox(_ptr(sky), 0 if sky is None else int(sky.shape[1]), 0)
+    rp = L.ReflectionParams(march, float(maxt), seed & 0xFFFFFFFF, frame)
+    v = view_struct(vd)
+    g = gbuf.struct()
+    check(L.lib().vx_reflection(ctx.h, vol.h, C.byref(v), C.byref(g), _ptr(palette), C.byref(io), C.byref(sb),
+                                C.byref(rp), _stream(stream)))
+
+
+def gbuf_depth(ctx: Context, vd: dict, gbuf: GBuffer, depth: torch.Tensor, volume_rid: int = 0, stream=None):
+    """The depth attachment (npx) of a G-buffer written by vx_primary."""
+    v = view_struct(vd)
+    g = gbuf.struct()
+    check(L.lib().vx_gbuf_depth(ctx.h, C.byref(v), C.byref(g), volume_rid, _ptr(depth), _stream(stream)))
+
+
+def ssao(ctx: Context, vd: dict, gbuf: GBuffer, depth: torch.Tensor, out: torch.Tensor, seed: int = 0, stream=None):
+    """calculateOcclusion per pixel into out (npx), from the G-buffer normals
+    and the depth attachment."""
+    v = view_struct(vd)
+    g = gbuf.struct()
+    check(L.lib().vx_ssao(ctx.h, C.byref(v), C.byref(g), _ptr(depth), seed & 0xFFFFFFFF, _ptr(out), _stream(stream)))

@@ -8,8 +8,9 @@ B  batched: 64 scenes, each axis a multiple of 8 in [64,512], fill 5..40%, one v
    from 256x256 to 1920x1080, 1 shadow + 8 AO rays.
 D  differentiable shading: 256^3 terrain, 1280x720, 255-entry fp32 palette, directional
    light, AO-weighted ambient, visibility fixed.
-L  large: 1024^3 city blocks (20000 grounded boxes), 3 coarse levels, four 3840x2160 views,
-   1 shadow + 32 AO + 1 spec-occ ray per pixel, forward + backward.
+L  large: 1024^3 city blocks (20000 grounded boxes, edges 8..56: 5.2% fill), 3 coarse levels,
+   four 3840x2160 views from inside the city (96% of the pixels see a block), 1 shadow + 32 AO +
+   1 spec-occ ray per pixel, forward + backward.
 All inputs are synthetic and seeded.
 """
 from __future__ import annotations
@@ -45,10 +46,18 @@ CONFIGS = {
     "D": dict(dims=(256, 256, 256), fill=0.30, nboxes=0, box=(2, 32), grounded=False, levels=2,
               width=1280, height=720, nviews=1,
               params=dict(n_shadow=1, n_ao=8, ao_maxt=32.0, n_spec=0, march="dda")),
-    "L": dict(dims=(1024, 1024, 1024), fill=0.0, nboxes=20000, box=(4, 24), grounded=True, levels=3,
-              width=3840, height=2160, nviews=4,
+    "L": dict(dims=(1024, 1024, 1024), fill=0.0, nboxes=20000, box=(8, 56), grounded=True, levels=3,
+              width=3840, height=2160, nviews=4, camera="city",
               params=dict(n_shadow=1, n_ao=32, ao_maxt=32.0, n_spec=1, march="dda")),
 }
+
+
+def config_view(cfg: dict, width=None, height=None, variant=0, frame=0) -> dict:
+    """The view of a config: inside the city for L, above a corner otherwise."""
+    w, h = width or cfg["width"], height or cfg["height"]
+    if cfg.get("camera") == "city":
+        return views.city_view(cfg["dims"], w, h, frame=frame, variant=variant)
+    return views.scene_view(cfg["dims"], w, h, frame=frame, variant=variant)
 
 
 def rays_per_pixel(params: dict) -> int:

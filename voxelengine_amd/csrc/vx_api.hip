@@ -99,8 +99,16 @@ static void march_buffers_free(MarchBuffers& m) {
     std::memset(&m, 0, sizeof(m));
 }
 
+// Persistent-grid size of one march kernel, cached per context: the device's
+// CU count and the kernel's occupancy do not change between launches.
+struct GridCache {
+    int cus;
+    uint32_t grid[2][2][3];   // [single][block 256/512][steps 2/4/8]: 0 = not yet asked
+};
+
 struct vx_ctx {
     int32_t device;
+    GridCache gcache;
     long long* grad_acc;     // 256*6+6 fixed-point gradient accumulator
     ShadeArgs* shade_items;  // batched adjoint: per-view arguments (device)
     uint32_t* shade_block0;  // and each view's first block (n + 1 entries)
@@ -117,6 +125,16 @@ struct vx_ctx {
     uint32_t tile_cap;
     void* taa_rec;           // LightTAA: packed per-pixel attributes (64 B each)
     uint32_t taa_cap;
+    // light pass ray list: per pixel first record and count, the total, the records
+    uint32_t* lt_first;
+    uint32_t* lt_count;
+    uint32_t* lt_total;
+    size_t lt_px_cap;
+    LightRec* lt_rec;
+    float* lt_term;
+    size_t lt_ray_cap;
+    uint16_t* lt_rfetch;
+    size_t lt_fetch_cap;
     int32_t* rb_aabb;        // rebuild: per-instance AABB slots
     size_t rb_aabb_cap;
     FootprintOp* rb_ops;     // rebuild: footprint operations
@@ -248,9 +266,13 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     }
     VX_REQUIRE(device >= 0 && device < n, "device %d out of range (0..%d)", device, n - 1);
     VX_HIP(hipSetDevice(device));
+    int cus = 0;
+    VX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     vx_ctx* c = new (std::nothrow) vx_ctx();
     if (!c) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
+    std::memset(&c->gcache, 0, sizeof(c->gcache));
+    c->gcache.cus = cus;
     c->grad_acc = nullptr;
     c->shade_items = nullptr;
     c->shade_block0 = nullptr;
@@ -266,6 +288,13 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     c->tile_cap = 0;
     c->taa_rec = nullptr;
     c->taa_cap = 0;
+    c->lt_first = c->lt_count = c->lt_total = nullptr;
+    c->lt_px_cap = 0;
+    c->lt_rec = nullptr;
+    c->lt_term = nullptr;
+    c->lt_ray_cap = 0;
+    c->lt_rfetch = nullptr;
+    c->lt_fetch_cap = 0;
     c->rb_aabb = nullptr;
     c->rb_aabb_cap = 0;
     c->rb_ops = nullptr;
@@ -294,6 +323,12 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     (void)hipFree(ctx->tile_lists);
     (void)hipFree(ctx->tile_counts);
     (void)hipFree(ctx->taa_rec);
+    (void)hipFree(ctx->lt_first);
+    (void)hipFree(ctx->lt_count);
+    (void)hipFree(ctx->lt_total);
+    (void)hipFree(ctx->lt_rec);
+    (void)hipFree(ctx->lt_term);
+    (void)hipFree(ctx->lt_rfetch);
     (void)hipFree(ctx->rb_aabb);
     (void)hipFree(ctx->rb_ops);
     profile_clear(ctx);
@@ -313,6 +348,9 @@ extern "C" vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning*
     VX_REQUIRE(t->refill >= 0 && t->refill <= 64, "tuning.refill outside 0..64");
     VX_REQUIRE(t->steps == 0 || t->steps == 2 || t->steps == 4 || t->steps == 8, "tuning.steps %d not 2/4/8",
                t->steps);
+    VX_REQUIRE(t->tile >= 0 && t->tile <= 2, "tuning.tile %d outside 0..2", t->tile);
+    VX_REQUIRE(t->order == 0 || t->order == 1, "tuning.order %d not 0/1", t->order);
+    VX_REQUIRE(t->lights == 0 || t->lights == 1, "tuning.lights %d not 0/1", t->lights);
     ctx->tuning = *t;
     return VX_OK;
 }
@@ -496,6 +534,15 @@ extern "C" vx_status vx_volume_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32
     v->gz = ((2 * sz + 2) >> 2) + 2;
     v->ngrid = (size_t)v->gx * v->gy * v->gz;
     v->grid = nullptr;
+    // The ray-parallel trace addresses the march grid with 32-bit word indices.
+    if (v->ngrid >= (size_t(1) << 32)) {
+        const size_t words = v->ngrid;
+        delete v;
+        return fail(VX_ERR_RANGE,
+                    "volume %dx%dx%d needs a march grid of %zu words; the trace indexes it with 32-bit word "
+                    "indices (at most 2^32 - 1 words: with 4096 x 4096 texel slices, at most 2040 slices)",
+                    sx, sy, sz, words);
+    }
     hipError_t e = hipMalloc(&v->cells, v->ncells * 8);
     if (e == hipSuccess) e = hipMemset(v->cells, 0, v->ncells * 8);
     for (int l = 0; l < nlevels && e == hipSuccess; ++l) {
@@ -1137,14 +1184,19 @@ extern "C" vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa
     return VX_OK;
 }
 
-static vx_status persistent_grid(vx_ctx* ctx, const void* kernel, int block, size_t lds, int want_per_cu,
-                                 uint32_t& grid) {
-    int cus = 0, per_cu = 0;
-    VX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds));
-    if (per_cu < 1) per_cu = 1;
-    if (want_per_cu > 0 && want_per_cu < per_cu) per_cu = want_per_cu;
-    grid = (uint32_t)(cus * per_cu);
+// Persistent grid of a march kernel: CUs x resident blocks per CU (at most
+// want_per_cu when that is set), a multiple of 8 so every XCD band has blocks.
+// The occupancy query runs once per kernel and context.
+static vx_status persistent_grid(vx_ctx* ctx, bool single, int block, int steps, int want_per_cu, uint32_t& grid) {
+    uint32_t& per_cu_c = ctx->gcache.grid[single ? 1 : 0][block == 512 ? 1 : 0][steps == 2 ? 0 : (steps == 4 ? 1 : 2)];
+    if (per_cu_c == 0) {
+        int per_cu = 0;
+        VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel(block, steps, single), block, 0));
+        per_cu_c = (uint32_t)std::max(per_cu, 1);
+    }
+    uint32_t per_cu = per_cu_c;
+    if (want_per_cu > 0 && (uint32_t)want_per_cu < per_cu) per_cu = (uint32_t)want_per_cu;
+    grid = (uint32_t)ctx->gcache.cus * per_cu;
     grid = (grid + 7u) & ~7u;
     return VX_OK;
 }
@@ -1176,7 +1228,18 @@ static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
     return VX_OK;
 }
 
-static MarchItem make_march_item(const vx_volume* vol, const TraceArgs& T, uint32_t tile0) {
+static TileGeom tile_geom(const vx_trace_tuning& tu, int32_t width, int32_t height) {
+    TileGeom g;
+    g.tw_shift = tu.tile == 1 ? 4u : (tu.tile == 2 ? 2u : 3u);
+    g.order = tu.order == 1 ? 1u : 0u;
+    g.tiles_x = cdiv((uint32_t)width, 1u << g.tw_shift);
+    g.tiles_y = cdiv((uint32_t)height, 64u >> g.tw_shift);
+    g.nbx = cdiv(g.tiles_x, 8);
+    g.ntiles = g.order == 0 ? g.tiles_x * g.tiles_y : g.nbx * cdiv(g.tiles_y, 8) * 64u;
+    return g;
+}
+
+static MarchItem make_march_item(const vx_volume* vol, const TraceArgs& T, uint32_t tile0, const TileGeom& g) {
     MarchItem I;
     std::memset(&I, 0, sizeof(I));
     I.V = occ_view(vol);
@@ -1185,9 +1248,12 @@ static MarchItem make_march_item(const vx_volume* vol, const TraceArgs& T, uint3
     I.gx = vol->gx;
     I.gxy = vol->gx * vol->gy;
     I.lim[0] = 2 * vol->sx + 1; I.lim[1] = 2 * vol->sy + 1; I.lim[2] = 2 * vol->sz + 1;
-    I.tiles_x = cdiv((uint32_t)T.view.width, 8);
-    I.ntiles = I.tiles_x * cdiv((uint32_t)T.view.height, 8);
+    I.tiles_x = g.tiles_x;
+    I.ntiles = g.ntiles;
     I.tile0 = tile0;
+    I.tw_shift = g.tw_shift;
+    I.order = g.order;
+    I.nbx = g.nbx;
     return I;
 }
 
@@ -1201,7 +1267,7 @@ static vx_status march_shape(vx_ctx* ctx, bool single, MarchShape& sh) {
     sh.block = tu.block == 512 ? 512 : 256;
     sh.steps = tu.steps ? tu.steps : 8;
     sh.rest_grid = 64;
-    return persistent_grid(ctx, march_kernel(sh.block, sh.steps, single), sh.block, 0, tu.blocks_per_cu, sh.grid);
+    return persistent_grid(ctx, single, sh.block, sh.steps, tu.blocks_per_cu, sh.grid);
 }
 
 static void march_common(MarchArgs& A, const vx_trace_params& p, const vx_trace_tuning& tu) {
@@ -1212,25 +1278,53 @@ static void march_common(MarchArgs& A, const vx_trace_params& p, const vx_trace_
     A.maxt[2] = p.spec_maxt;
 }
 
-static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T, hipStream_t stream) {
+// Scratch of one single-view march launch: per-slot results, listed rays,
+// fetch counts (may be null) and the queue counters.
+struct MarchScratch {
+    float* res;
+    uint32_t* deferred;
+    uint16_t* fres;
+    uint32_t* counters;
+};
+
+// Enqueues the ray-parallel trace of the view's 8x8 tiles [tile_begin,
+// tile_begin + ntiles) (counter clear, march, listed rays, resolve); slots
+// count from the first of those tiles. e0 / e1 (optional) bracket the march.
+static vx_status enqueue_trace_march(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T, uint32_t tile_begin,
+                                     uint32_t ntiles, const MarchScratch& M, hipStream_t stream, hipEvent_t e0,
+                                     hipEvent_t e1, vx_trace_hit* hits = nullptr) {
     MarchArgs A;
     std::memset(&A, 0, sizeof(A));
-    A.item0 = make_march_item(vol, T, 0);
+    // tile0 = -tile_begin (mod 2^32): launch tile t is view tile t + tile_begin
+    A.item0 = make_march_item(vol, T, 0u - tile_begin, tile_geom(ctx->tuning, T.view.width, T.view.height));
     march_common(A, T.params, ctx->tuning);
     A.items = nullptr;
     A.nitems = 1;
-    A.ntiles = A.item0.ntiles;
+    A.ntiles = ntiles;
     A.band = cdiv(A.ntiles, 8);
     const size_t slots = (size_t)A.ntiles * (size_t)A.nrays * 64;
-    VX_REQUIRE(slots < (1ull << 32), "view needs %zu ray slots, more than 2^32", slots);
-    if (vx_status st = ensure_march(ctx->march, slots, T.fetches != nullptr)) return st;
-    A.counters = ctx->march.counters;
-    A.deferred = ctx->march.deferred;
+    A.counters = M.counters;
+    A.deferred = M.deferred;
     A.deferred_cap = (uint32_t)slots;
-    A.res = ctx->march.res;
-    A.fres = T.fetches ? ctx->march.fres : nullptr;
+    A.res = M.res;
+    A.fres = T.fetches ? M.fres : nullptr;
+    A.hits = hits;
     MarchShape sh;
     if (vx_status st = march_shape(ctx, true, sh)) return st;
+    hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(kCounterWords, 256)), dim3(256), 0, stream, A.counters, (int)kCounterWords);
+    hipError_t e = launch_march(sh.block, sh.steps, true, sh.grid, sh.rest_grid, stream, A, e0, e1);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of the ray-parallel trace failed: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T, hipStream_t stream,
+                                    vx_trace_hit* hits = nullptr) {
+    const uint32_t ntiles = tile_geom(ctx->tuning, T.view.width, T.view.height).ntiles;
+    const int32_t nrays = T.params.n_shadow + T.params.n_ao + T.params.n_spec;
+    const size_t slots = (size_t)ntiles * (size_t)nrays * 64;
+    VX_REQUIRE(slots < (1ull << 32), "view needs %zu ray slots, more than 2^32", slots);
+    if (vx_status st = ensure_march(ctx->march, slots, T.fetches != nullptr)) return st;
+    MarchScratch M = {ctx->march.res, ctx->march.deferred, ctx->march.fres, ctx->march.counters};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->profile) {
         VX_HIP(hipEventCreate(&e0));
@@ -1240,10 +1334,7 @@ static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tra
         }
         ctx->prof.emplace_back(e0, e1);
     }
-    hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(kCounterWords, 256)), dim3(256), 0, stream, A.counters, (int)kCounterWords);
-    hipError_t e = launch_march(sh.block, sh.steps, true, sh.grid, sh.rest_grid, stream, A, e0, e1);
-    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of the ray-parallel trace failed: %s", hipGetErrorString(e));
-    return VX_OK;
+    return enqueue_trace_march(ctx, vol, T, 0, ntiles, M, stream, e0, e1, hits);
 }
 
 static vx_status check_params(const vx_trace_params* p) {
@@ -1257,9 +1348,9 @@ static vx_status check_params(const vx_trace_params* p) {
     return VX_OK;
 }
 
-extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
-                              const float* palette, const vx_trace_params* params, const vx_trace_out* out,
-                              void* stream) {
+static vx_status trace_impl(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                            const float* palette, const vx_trace_params* params, const vx_trace_out* out,
+                            void* stream, vx_trace_hit* hits = nullptr) {
     VX_REQUIRE(ctx && vol && g && out && palette, "ctx/volume/gbuf/out/palette is NULL");
     if (vx_status st = check_view(view)) return st;
     if (vx_status st = check_params(params)) return st;
@@ -1276,10 +1367,36 @@ extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* 
     const bool all_dda = (params->n_shadow == 0 || params->shadow_march == VX_MARCH_DDA) &&
                          (params->n_ao == 0 || params->ao_march == VX_MARCH_DDA) &&
                          (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA);
-    if (all_dda && ctx->tuning.kernel != 1) return launch_trace_march(ctx, vol, A, S(stream));
+    if (hits) {
+        VX_REQUIRE(all_dda, "hit records are written by the DDA trace: every ray class must use VX_MARCH_DDA");
+        VX_REQUIRE(((uintptr_t)hits & 7u) == 0, "hits must be 8-byte aligned");
+        const size_t n64 = (size_t)view->width * view->height * (size_t)(params->n_shadow + params->n_ao +
+                                                                         params->n_spec) * (sizeof(vx_trace_hit) / 8);
+        if (n64) {
+            hipLaunchKernelGGL(k_fill_u64, dim3(cdiv(n64, 256)), dim3(256), 0, S(stream), reinterpret_cast<uint64_t*>(hits),
+                               n64, 0ull);
+            VX_LAUNCHED("k_fill_u64");
+        }
+        return launch_trace_march(ctx, vol, A, S(stream), hits);
+    }
+    if (all_dda && ctx->tuning.kernel != 1)
+        return launch_trace_march(ctx, vol, A, S(stream));
     hipLaunchKernelGGL(k_trace, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
     VX_LAUNCHED("k_trace");
     return VX_OK;
+}
+
+extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                              const float* palette, const vx_trace_params* params, const vx_trace_out* out,
+                              void* stream) {
+    return trace_impl(ctx, vol, view, g, palette, params, out, stream);
+}
+
+extern "C" vx_status vx_trace_hits(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                                   const float* palette, const vx_trace_params* params, const vx_trace_out* out,
+                                   vx_trace_hit* hits, void* stream) {
+    VX_REQUIRE(hits, "hits is NULL");
+    return trace_impl(ctx, vol, view, g, palette, params, out, stream, hits);
 }
 
 extern "C" vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const float* rays, uint32_t n, int32_t march,
@@ -1292,6 +1409,71 @@ extern "C" vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const floa
     if (vx_status st = use_device(ctx)) return st;
     hipLaunchKernelGGL(k_trace_rays, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), occ_view(vol), rays, n, march, out);
     VX_LAUNCHED("k_trace_rays");
+    return VX_OK;
+}
+
+// Grows a device array to n elements (contents not kept); work queued on the
+// device that may still read the old one is waited for first.
+template <class T>
+static vx_status grow(T*& p, size_t& cap, size_t n) {
+    if (cap >= n) return VX_OK;
+    VX_HIP(hipDeviceSynchronize());
+    (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    VX_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    cap = n;
+    return VX_OK;
+}
+
+// The light pass as a list of (pixel, light) shadow rays: count (one atomic
+// per 16x16 tile), read the total back, list, march one ray per lane, fold
+// each pixel's terms in draw order. Blocks until the total is known.
+static vx_status light_pass_list(vx_ctx* ctx, const vx_volume* vol, const LightArgs& A, hipStream_t stream) {
+    const size_t npx = (size_t)A.view.width * A.view.height;
+    size_t one = ctx->lt_total ? 1 : 0;
+    if (vx_status st = grow(ctx->lt_total, one, 1)) return st;
+    size_t pxc = ctx->lt_px_cap;
+    if (vx_status st = grow(ctx->lt_first, pxc, npx)) return st;
+    pxc = ctx->lt_px_cap;
+    if (vx_status st = grow(ctx->lt_count, pxc, npx)) return st;
+    ctx->lt_px_cap = std::max(ctx->lt_px_cap, npx);
+    LightListArgs B;
+    B.total = ctx->lt_total;
+    B.first = ctx->lt_first;
+    B.count = ctx->lt_count;
+    B.rec = nullptr;
+    B.term = nullptr;
+    B.rfetch = nullptr;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, B.total, 1);
+    hipLaunchKernelGGL(k_light_count, dim3(8 * A.tiles.per_xcd), dim3(256), 0, stream, A, B);
+    VX_LAUNCHED("k_light_count");
+    uint32_t total = 0;
+    VX_HIP(hipMemcpyAsync(&total, B.total, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    VX_HIP(hipStreamSynchronize(stream));
+    if (total == 0) {
+        hipLaunchKernelGGL(k_light_fold, dim3(cdiv(npx, 256)), dim3(256), 0, stream, A, B);
+        VX_LAUNCHED("k_light_fold");
+        return VX_OK;
+    }
+    size_t rc = ctx->lt_ray_cap;
+    if (vx_status st = grow(ctx->lt_rec, rc, total)) return st;
+    rc = ctx->lt_ray_cap;
+    if (vx_status st = grow(ctx->lt_term, rc, 3 * (size_t)total)) return st;
+    ctx->lt_ray_cap = std::max<size_t>(ctx->lt_ray_cap, total);
+    if (A.fetches) {
+        if (vx_status st = grow(ctx->lt_rfetch, ctx->lt_fetch_cap, total)) return st;
+    }
+    B.rec = ctx->lt_rec;
+    B.term = ctx->lt_term;
+    B.rfetch = A.fetches ? ctx->lt_rfetch : nullptr;
+    hipLaunchKernelGGL(k_light_records, dim3(8 * A.tiles.per_xcd), dim3(256), 0, stream, A, B);
+    VX_LAUNCHED("k_light_records");
+    const uint32_t grid = std::min<uint32_t>(cdiv(total, 256), (uint32_t)ctx->gcache.cus * 8u);
+    hipLaunchKernelGGL(k_light_march, dim3(grid), dim3(256), 0, stream, occ_view(vol), A, B);
+    VX_LAUNCHED("k_light_march");
+    hipLaunchKernelGGL(k_light_fold, dim3(cdiv(npx, 256)), dim3(256), 0, stream, A, B);
+    VX_LAUNCHED("k_light_fold");
     return VX_OK;
 }
 
@@ -1316,8 +1498,80 @@ extern "C" vx_status vx_light_pass(vx_ctx* ctx, const vx_volume* vol, const vx_v
     A.params = *params;
     A.rgb = rgb;
     A.fetches = fetches;
-    hipLaunchKernelGGL(k_lights, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
-    VX_LAUNCHED("k_lights");
+    if (ctx->tuning.lights == 1 || nlights == 0) {
+        hipLaunchKernelGGL(k_lights, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
+        VX_LAUNCHED("k_lights");
+        return VX_OK;
+    }
+    return light_pass_list(ctx, vol, A, S(stream));
+}
+
+extern "C" vx_status vx_reflection(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                                   const float* palette, const vx_reflection_io* io, const vx_skybox* sky,
+                                   const vx_reflection_params* params, void* stream) {
+    VX_REQUIRE(ctx && vol && g && palette && io && params, "ctx/volume/gbuf/palette/io/params is NULL");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g->pos && g->normal && g->material, "gbuf has a NULL pointer");
+    VX_REQUIRE(io->light && io->depth && io->out, "io: light, depth and out are required");
+    VX_REQUIRE(((uintptr_t)io->out & 15u) == 0 && ((uintptr_t)io->light & 3u) == 0, "io.out must be 16-byte aligned");
+    VX_REQUIRE(params->march == VX_MARCH_DDA || params->march == VX_MARCH_SPARSE ||
+                   params->march == VX_MARCH_SUPERSPARSE, "unknown marcher %d", params->march);
+    VX_REQUIRE(params->maxt > 0.0f, "maxt must be positive");
+    VX_REQUIRE(!sky || !sky->faces || (sky->size >= 1 && sky->size <= 16384 && ((uintptr_t)sky->faces & 15u) == 0),
+               "skybox: size 1..16384 and 16-byte aligned faces");
+    if (vx_status st = use_device(ctx)) return st;
+    ReflectionArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.view = *view;
+    A.tiles = tile_map(view->width, view->height);
+    A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
+    A.palette = palette;
+    A.depth = io->depth;
+    A.light = io->light;
+    A.sky = (sky && sky->faces) ? sky->faces : nullptr;
+    A.sky_size = A.sky ? sky->size : 0;
+    A.params = *params;
+    A.out = io->out;
+    A.t = io->t;
+    hipLaunchKernelGGL(k_reflection, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
+    VX_LAUNCHED("k_reflection");
+    return VX_OK;
+}
+
+extern "C" vx_status vx_ssao(vx_ctx* ctx, const vx_view* view, const vx_gbuf* g, const float* depth, uint32_t seed,
+                             float* ssao, void* stream) {
+    VX_REQUIRE(ctx && g && depth && ssao, "ctx/gbuf/depth/ssao is NULL");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g->normal && g->material, "gbuf normal/material is NULL");
+    if (vx_status st = use_device(ctx)) return st;
+    SsaoArgs A;
+    A.view = *view;
+    A.nrm = g->normal;
+    A.mat = g->material;
+    A.depth = depth;
+    A.seed = seed;
+    A.out = ssao;
+    const uint32_t npx = (uint32_t)view->width * (uint32_t)view->height;
+    hipLaunchKernelGGL(k_ssao, dim3(cdiv(npx, 256)), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_ssao");
+    return VX_OK;
+}
+
+extern "C" vx_status vx_gbuf_depth(vx_ctx* ctx, const vx_view* view, const vx_gbuf* g, int32_t volume_rid,
+                                   float* depth, void* stream) {
+    VX_REQUIRE(ctx && g && depth, "ctx/gbuf/depth is NULL");
+    if (vx_status st = check_view(view)) return st;
+    VX_REQUIRE(g->pos && g->material, "gbuf pos/material is NULL");
+    if (vx_status st = use_device(ctx)) return st;
+    DepthArgs A;
+    mat_mul_f32(view->proj, view->view, A.pv);
+    A.pos = g->pos;
+    A.mat = g->material;
+    A.depth = depth;
+    A.npx = (uint32_t)view->width * (uint32_t)view->height;
+    A.rid_scale = 1.0f - 1e-7f * (float)volume_rid;
+    hipLaunchKernelGGL(k_gbuf_depth, dim3(cdiv(A.npx, 256)), dim3(256), 0, S(stream), A);
+    VX_LAUNCHED("k_gbuf_depth");
     return VX_OK;
 }
 
@@ -1477,7 +1731,7 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
         T.palette = palette;
         T.shadow = it.out.shadow; T.ao = it.out.ao; T.spec = it.out.spec; T.fetches = it.out.fetches;
         fetches = fetches || it.out.fetches;
-        h[i] = make_march_item(it.volume, T, (uint32_t)tiles);
+        h[i] = make_march_item(it.volume, T, (uint32_t)tiles, tile_geom(ctx->tuning, it.view.width, it.view.height));
         tiles += h[i].ntiles;
         VX_REQUIRE(tiles < (1ull << 31), "batch too large");
     }

@@ -43,8 +43,16 @@ struct MarchItem {
     const uint64_t* grid;      // march grid (border cells included, words bit-reversed)
     int32_t gx, gxy;           // march grid strides in words
     int32_t lim[3];            // 2*s+1: last in-bounds fine voxel per axis
-    uint32_t tiles_x, ntiles;  // 8x8 tiles of the view
+    uint32_t tiles_x, ntiles;  // tiles of the view per row; queue entries (padded to whole blocks in order 1)
     uint32_t tile0;            // first global tile
+    uint32_t tw_shift;         // tile width 1 << tw_shift pixels (3: 8x8, 4: 16x4, 2: 4x16)
+    uint32_t order;            // 0: rows of tiles; 1: rows of 8x8-tile blocks, row-major inside a block
+    uint32_t nbx;              // blocks per row (order 1)
+};
+
+// Tile geometry of a view for the ray-parallel trace under a tuning.
+struct TileGeom {
+    uint32_t tw_shift, order, tiles_x, tiles_y, nbx, ntiles;
 };
 
 // Queue counters of the ray-parallel trace, one 128-byte line each: 8 band
@@ -63,6 +71,7 @@ struct MarchArgs {
     uint32_t deferred_cap;     // capacity of deferred (every slot fits)
     float* res;                // per-ray best_t of a hit, a marker NaN otherwise; slot = ((tile * nrays + r) << 6) | pixel-in-tile
     uint16_t* fres;            // per-ray fetch count, or null
+    vx_trace_hit* hits;        // per (pixel, ray) hit records of a single-view launch, or null
     int32_t nrays;
     int32_t refill;            // not-running lanes that trigger a refill
     float maxt[3];             // shadow, AO, spec
@@ -80,6 +89,58 @@ struct LightArgs {
     vx_light_params params;
     float* rgb;
     uint32_t* fetches;
+};
+
+// LightReflection (LightReflection.frag:60-139).
+struct ReflectionArgs {
+    vx_view view;
+    TileMap tiles;
+    const float* pos;
+    const float* nrm;
+    const uint8_t* mat;
+    const float* palette;
+    const float* depth;        // D24 depth per pixel (1 = sky)
+    const float* light;        // 4 per pixel (lightDiffuse)
+    const float* sky;          // cube faces (6 * size^2 * 4) or null: the procedural sky
+    int32_t sky_size;
+    vx_reflection_params params;
+    float* out;                // 4 per pixel
+    float* t;                  // per pixel or null
+};
+
+// Screen-space occlusion (LightAmbient.frag:59-109).
+struct SsaoArgs {
+    vx_view view;
+    const float* nrm;
+    const uint8_t* mat;
+    const float* depth;
+    uint32_t seed;
+    float* out;
+};
+
+// Per-pixel D24 depth of a G-buffer (vx_gbuf_depth).
+struct DepthArgs {
+    float pv[16];              // Projection * View
+    const float* pos;
+    const uint8_t* mat;
+    float* depth;
+    uint32_t npx;
+    float rid_scale;           // 1 - 1e-7 * VolumeRID
+};
+
+// The light pass's ray list (k_light_count / _records / _march / _fold).
+struct LightRec {
+    uint32_t pixel;
+    uint32_t light;
+};
+
+struct LightListArgs {
+    uint32_t* total;           // rays listed (one atomic per tile)
+    uint32_t* first;           // per pixel: its first record
+    uint32_t* count;           // per pixel: its in-range lights
+    LightRec* rec;             // the records, a pixel's in draw order
+    float* term;               // 3 per record: the light's shadowed direct term
+    uint16_t* rfetch;          // per record: the marcher's fetch count, or null
 };
 
 // A model's three mips (X-fastest bytes) and mip-0 dims.
@@ -187,7 +248,14 @@ __global__ void k_trace_rays(OccView V, const float* rays, uint32_t n, int32_t m
 __global__ void __launch_bounds__(256) k_trace(OccView V, TraceArgs A);
 __global__ void k_zero_u32(uint32_t* p, int n);
 __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A);
+__global__ void __launch_bounds__(256) k_light_count(LightArgs A, LightListArgs B);
+__global__ void __launch_bounds__(256) k_light_records(LightArgs A, LightListArgs B);
+__global__ void __launch_bounds__(256) k_light_march(OccView V, LightArgs A, LightListArgs B);
+__global__ void __launch_bounds__(256) k_light_fold(LightArgs A, LightListArgs B);
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A);
+__global__ void __launch_bounds__(256) k_reflection(OccView V, ReflectionArgs A);
+__global__ void __launch_bounds__(256) k_gbuf_depth(DepthArgs A);
+__global__ void __launch_bounds__(256) k_ssao(SsaoArgs A);
 __global__ void __launch_bounds__(256) k_bin_instances(BinArgs A);
 __global__ void __launch_bounds__(256) k_primary_inst(PrimaryInstArgs A);
 __global__ void __launch_bounds__(256) k_taa_pack(TaaArgs A);

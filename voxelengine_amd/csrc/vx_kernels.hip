@@ -786,6 +786,31 @@ VXD float best_t_of(float tm0, float tm1, float tm2) {
 }
 
 template <bool SINGLE>
+VXD const MarchItem& march_item(const MarchArgs& A, uint32_t k);
+template <bool SINGLE>
+VXD uint32_t item_of_tile(const MarchArgs& A, uint32_t gt);
+VXD bool tile_lane_pixel(const MarchItem& I, uint32_t lt, uint32_t l, uint32_t& i);
+
+// Hit record of the ray in `slot` (A.hits[pixel * nrays + r]).
+template <bool SINGLE>
+VXD void put_hit(const MarchArgs& A, uint32_t slot, int32_t term, int32_t steps, int32_t fetches,
+                 const int32_t vox[3], float t, const float nrm[3]) {
+    const uint32_t q = slot >> 6, l = slot & 63u;
+    const uint32_t gt = q / (uint32_t)A.nrays, r = q - gt * (uint32_t)A.nrays;
+    const MarchItem& I = march_item<SINGLE>(A, item_of_tile<SINGLE>(A, gt));
+    uint32_t i = 0;
+    if (!tile_lane_pixel(I, gt - I.tile0, l, i)) return;
+    vx_trace_hit h;
+    h.term = term;
+    h.steps = steps;
+    h.fetches = fetches;
+    h.voxel[0] = vox[0]; h.voxel[1] = vox[1]; h.voxel[2] = vox[2];
+    h.t = t;
+    h.normal[0] = nrm[0]; h.normal[1] = nrm[1]; h.normal[2] = nrm[2];
+    A.hits[(size_t)i * (uint32_t)A.nrays + r] = h;
+}
+
+template <bool SINGLE>
 VXD const MarchItem& march_item(const MarchArgs& A, uint32_t k) {
     if (SINGLE) return A.item0;
     return A.items[k];
@@ -824,11 +849,24 @@ VXD uint32_t item_of_tile(const MarchArgs& A, uint32_t gt) {
     return lo;
 }
 
-// Pixel index of lane l of tile lt, or false outside the view.
+// Pixel index of lane l of queue tile lt, or false outside the view. A tile
+// is (1 << tw_shift) x (64 >> tw_shift) pixels, lanes row-major inside it; in
+// order 1 the queue walks 8x8-tile blocks row by row, tiles row-major inside
+// a block (the padding tiles of the last block column / row lie outside).
 VXD bool tile_lane_pixel(const MarchItem& I, uint32_t lt, uint32_t l, uint32_t& i) {
-    uint32_t px = (lt % I.tiles_x) * 8u + (l & 7u), py = (lt / I.tiles_x) * 8u + (l >> 3);
+    uint32_t tx, ty;
+    if (I.order == 0) {
+        tx = lt % I.tiles_x;
+        ty = lt / I.tiles_x;
+    } else {
+        const uint32_t b = lt >> 6, w = lt & 63u;
+        tx = (b % I.nbx) * 8u + (w & 7u);
+        ty = (b / I.nbx) * 8u + (w >> 3);
+    }
+    const uint32_t px = (tx << I.tw_shift) + (l & ((1u << I.tw_shift) - 1u));
+    const uint32_t py = ty * (64u >> I.tw_shift) + (l >> I.tw_shift);
     i = py * (uint32_t)I.T.view.width + px;
-    return px < (uint32_t)I.T.view.width && py < (uint32_t)I.T.view.height;
+    return tx < I.tiles_x && px < (uint32_t)I.T.view.width && py < (uint32_t)I.T.view.height;
 }
 
 // Origin voxel and DDA state of ray (o, d) at the origin, Light.frag:32-46
@@ -956,6 +994,21 @@ k_march(MarchArgs A) {
                 }
                 A.res[R.slot] = v;
                 if (A.fres) A.fres[R.slot] = (uint16_t)ray_fetches(term, (int32_t)n);
+                if (COUNT && A.hits) {
+                    // Light.frag:57-63 at the hit voxel: normal = -stepSign * select
+                    int32_t vox[3] = {0, 0, 0};
+                    float nrm[3] = {0.0f, 0.0f, 0.0f}, tt = 0.0f;
+                    if (term == 1) {
+                        vox[0] = R.x - 4; vox[1] = (R.y >> 2) - 4; vox[2] = (R.z >> 4) - 4;
+                        tt = v;
+                        const float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
+                        nrm[0] = -(R.sx > 0 ? 1.0f : -1.0f) * (R.tm0 == tmin ? 1.0f : 0.0f);
+                        nrm[1] = -(R.sy > 0 ? 1.0f : -1.0f) * (R.tm1 == tmin ? 1.0f : 0.0f);
+                        nrm[2] = -(R.sz > 0 ? 1.0f : -1.0f) * (R.tm2 == tmin ? 1.0f : 0.0f);
+                    }
+                    put_hit<SINGLE>(A, R.slot, term, term == 0 ? 4 * (int32_t)n : (int32_t)n,
+                                    (int32_t)ray_fetches(term, (int32_t)n), vox, tt, nrm);
+                }
                 hasray = false;
                 stopped = false;
             }
@@ -1022,6 +1075,11 @@ k_march(MarchArgs A) {
                             // starts outside: term 2 at step 0
                             A.res[slot] = __uint_as_float(kNoHitBits);
                             if (A.fres) A.fres[slot] = 0;
+                            if (COUNT && A.hits) {
+                                const int32_t z3[3] = {0, 0, 0};
+                                const float zf[3] = {0.0f, 0.0f, 0.0f};
+                                put_hit<SINGLE>(A, slot, 2, 0, 0, z3, 0.0f, zf);
+                            }
                         }
                     }
                 }
@@ -1129,9 +1187,18 @@ __global__ void __launch_bounds__(256) k_march_rest(MarchArgs A) {
         float maxt;
         (void)pixel_ray(I.T, P, wd, (int32_t)r, o, d, maxt);
         RayOut R;
+        R.vox[0] = R.vox[1] = R.vox[2] = 0;
+        R.t = 0.0f;
+        R.normal = mk3(0.0f, 0.0f, 0.0f);
         dda_exact(I.V, o, d, maxt, R);
         A.res[slot] = R.hit ? R.t : __uint_as_float(kNoHitBits);
         if (A.fres) A.fres[slot] = (uint16_t)R.fetches;
+        if (A.hits) {
+            const bool h = R.hit != 0;
+            const int32_t vox[3] = {h ? R.vox[0] : 0, h ? R.vox[1] : 0, h ? R.vox[2] : 0};
+            const float nrm[3] = {h ? R.normal.x : 0.0f, h ? R.normal.y : 0.0f, h ? R.normal.z : 0.0f};
+            put_hit<SINGLE>(A, slot, R.term, R.steps, R.fetches, vox, h ? R.t : 0.0f, nrm);
+        }
     }
 }
 
@@ -1196,8 +1263,8 @@ __global__ void __launch_bounds__(256) k_resolve(MarchArgs A) {
 // Launchers (template instances live in this file).
 template <int BLOCK, int STEPS>
 static void launch_march_t(bool single, uint32_t grid, hipStream_t s, const MarchArgs& A) {
-    // the per-step count only where the fetch counts are asked for
-    const bool count = A.fres != nullptr;
+    // the per-step count only where fetch counts or hit records are asked for
+    const bool count = A.fres != nullptr || A.hits != nullptr;
     if (single && count) hipLaunchKernelGGL((k_march<BLOCK, STEPS, true, true>), dim3(grid), dim3(BLOCK), 0, s, A);
     else if (single) hipLaunchKernelGGL((k_march<BLOCK, STEPS, true, false>), dim3(grid), dim3(BLOCK), 0, s, A);
     else if (count) hipLaunchKernelGGL((k_march<BLOCK, STEPS, false, true>), dim3(grid), dim3(BLOCK), 0, s, A);
@@ -1296,24 +1363,13 @@ __global__ void k_mgrid(uint64_t* grid, OccView V, int32_t gx, int32_t gy, int32
 // ===========================================================================
 constexpr uint32_t kMaxLights = 4096;
 
-__global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
-    __shared__ uint16_t list_s[kMaxLights];
-    __shared__ float box_s[4][6];
-    __shared__ uint32_t wcount_s[4];
-    __shared__ uint32_t bad_s;
-    const uint32_t logical = (blockIdx.x % 8u) * A.tiles.per_xcd + blockIdx.x / 8u;
-    if (logical >= A.tiles.ntiles) return;   // block-uniform
+// The lights of a 16x16 block tile that can reach its hit pixels, in draw
+// order, into list_s; returns their count. Every thread of the block calls it
+// (it synchronizes the block). valid / wpos: this thread's pixel.
+VXD uint32_t tile_light_list(const LightArgs& A, bool valid, f3 wpos, uint16_t* list_s, float (*box_s)[6],
+                             uint32_t* wcount_s, uint32_t* bad_s) {
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    int32_t px, py;
-    const bool inview = tile_pixel(A.tiles, px, py);
-    const size_t i = inview ? (size_t)py * A.view.width + px : 0;
-    const uint32_t m = inview ? A.mat[i] : 0u;
-    const bool valid = m != 0;
-    f3 pp = mk3(0.0f, 0.0f, 0.0f);
-    if (valid) pp = load3(A.pos + 3 * i);
-    const f3 wpos = mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f);
-    // the tile's box of world positions
-    if (t == 0) bad_s = 0;
+    if (t == 0) *bad_s = 0;
     const bool finite = isfinite(wpos.x) && isfinite(wpos.y) && isfinite(wpos.z);
     float lo[3] = {valid ? wpos.x : INFINITY, valid ? wpos.y : INFINITY, valid ? wpos.z : INFINITY};
     float hi[3] = {valid ? wpos.x : -INFINITY, valid ? wpos.y : -INFINITY, valid ? wpos.z : -INFINITY};
@@ -1329,7 +1385,7 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) { box_s[wave][k] = lo[k]; box_s[wave][3 + k] = hi[k]; }
-        if (wave_bad) bad_s = 1;
+        if (wave_bad) *bad_s = 1;
     }
     __syncthreads();
     float blo[3], bhi[3];
@@ -1338,9 +1394,8 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
         blo[k] = fminf(fminf(box_s[0][k], box_s[1][k]), fminf(box_s[2][k], box_s[3][k]));
         bhi[k] = fmaxf(fmaxf(box_s[0][3 + k], box_s[1][3 + k]), fmaxf(box_s[2][3 + k], box_s[3][3 + k]));
     }
-    const bool keep_all = bad_s != 0;
-    // the lights that can reach the box, in draw order: 256 at a time, each
-    // wave's survivors after those of the waves before it
+    const bool keep_all = *bad_s != 0;
+    // 256 lights at a time, each wave's survivors after those of the waves before it
     uint32_t n = 0;
     for (uint32_t c0 = 0; c0 < A.nlights; c0 += 256) {
         const uint32_t l = c0 + t;
@@ -1365,77 +1420,258 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
         n += (wcount_s[0] + wcount_s[1]) + (wcount_s[2] + wcount_s[3]);
         __syncthreads();   // counts read, list written
     }
-    if (!inview) return;
-    if (!valid) {
-        if (A.fetches) A.fetches[i] = 0;
-        return;
-    }
-    const f3 N = load3(A.nrm + 3 * i);
+    return n;
+}
+
+// A hit pixel's inputs to every light's term (LightPoint.frag:95-108,
+// LightSpot.frag:83-96; PBRDirectLight's F and kD, PBR.frag:48-59).
+struct LightPix {
+    f3 wpos, N, rv;
+    float r3, kD;
+    const float* e;
+};
+
+VXD void light_pixel_setup(const LightArgs& A, size_t i, LightPix& P) {
+    const f3 pp = load3(A.pos + 3 * i);
+    P.wpos = mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f);
+    P.N = load3(A.nrm + 3 * i);
     const uint32_t pk = pixkey(A.params.seed, A.params.frame, (uint32_t)i);
     float r0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) r0[k] = rand01(pk, (uint32_t)k);
     const f3 cam = mk3(A.view.cam_pos[0] * 10.0f, A.view.cam_pos[1] * 10.0f, A.view.cam_pos[2] * 10.0f);
     const f3 Vd = normalize3(mk3(cam.x - pp.x, cam.y - pp.y, cam.z - pp.z));
-    const float* e = A.palette + 6 * m;
-    const float metal = e[4];
-    const float ndv = gmax(dot3(N, Vd), 0.0f);
+    P.e = A.palette + 6 * (uint32_t)A.mat[i];
+    const float metal = P.e[4];
+    const float ndv = gmax(dot3(P.N, Vd), 0.0f);
     const float x = gmax(1.0f - ndv, 0.0f);
     const float s5 = (((x * x) * x) * x) * x;
     // PBRDirectLight (PBR.frag:48-68): F0 = mix(0.04, 1, metallic)
     const float f0d = 0.04f * (1.0f - metal) + 1.0f * metal;
     const float F = f0d + (1.0f - f0d) * s5;
-    const float kD = (1.0f - F) * (1.0f - metal);
+    P.kD = (1.0f - F) * (1.0f - metal);
     f3 rv = cosine_hemisphere(r0[0], r0[1]);
     rv = mk3(rv.x * 0.1f, rv.y * 0.1f, rv.z * 0.1f);
     rv.z = rv.z * gsign(r0[2] - 0.5f);
+    P.rv = rv;
+    P.r3 = r0[3];
+}
+
+// The range test of a light draw (LightPoint.frag:99-100): false outside.
+VXD bool light_in_range(const vx_light& L, f3 wpos, f3& ld, float& dist) {
+    ld = mk3(L.position[0] - wpos.x, L.position[1] - wpos.y, L.position[2] - wpos.z);
+    dist = sqrtf(dot3(ld, ld));
+    return !(dist > L.range);
+}
+
+// The light's shadow ray (LightPoint.frag:110-125, LightSpot.frag:98-113).
+VXD void light_ray(const vx_light& L, const LightPix& P, f3 ld, float dist, f3& o, f3& wdn, float& hit_dist) {
+    const bool spot = L.type == VX_LIGHT_SPOT;
+    f3 wd;
+    if (spot) {
+        const f3 nl = normalize3(ld);
+        wd = mk3(nl.x * 10.0f, nl.y * 10.0f, nl.z * 10.0f);
+        hit_dist = dist * 10.0f;
+    } else {
+        wd = ld;
+        hit_dist = dist * 10.5f;
+    }
+    const f3 rv = P.rv;
+    wdn = normalize3(mk3(wd.x * (1.0f - 0.5f) + rv.x * 0.5f, wd.y * (1.0f - 0.5f) + rv.y * 0.5f,
+                         wd.z * (1.0f - 0.5f) + rv.z * 0.5f));
+    float c0 = P.wpos.x * 10.0f, c1 = P.wpos.y * 10.0f, c2 = P.wpos.z * 10.0f;
+    c0 = c0 + wdn.x * (P.r3 * 1.0f); c1 = c1 + wdn.y * (P.r3 * 1.0f); c2 = c2 + wdn.z * (P.r3 * 1.0f);
+    c0 = c0 + rv.x * 2.5f; c1 = c1 + rv.y * 2.5f; c2 = c2 + rv.z * 2.5f;
+    o = mk3(c0 + P.N.x * 0.5f, c1 + P.N.y * 0.5f, c2 + P.N.z * 0.5f);
+}
+
+// The light's shadowed direct term per channel (LightPoint.frag:127-150,
+// LightSpot.frag:115-139 with PBRDirectLight): the value the reference's
+// additive blending adds to the light buffer.
+VXD void light_term(const vx_light& L, const LightPix& P, f3 ld, float dist, float shadow, float term[3]) {
+    float att = gclamp(L.range - dist, 0.0f, 1.0f) / pow_vx(dist, L.attenuation);
+    if (L.type == VX_LIGHT_SPOT) {
+        const f3 nl = normalize3(ld);
+        const f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
+        const float ad = gmax(dot3(nl, dir) - (1.0f - L.angle), 0.0f) / L.angle;
+        att = (pow_vx(ad, L.angle_attenuation) * gclamp(L.range - dist, 0.0f, 1.0f)) / pow_vx(dist, L.attenuation);
+    }
+    const float ndl = gmax(dot3(P.N, ld), 0.0f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float rad = (L.color[c] * att) * shadow;
+        term[c] = (((P.kD * P.e[c]) / 3.14159265359f) * rad) * ndl;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
+    __shared__ uint16_t list_s[kMaxLights];
+    __shared__ float box_s[4][6];
+    __shared__ uint32_t wcount_s[4];
+    __shared__ uint32_t bad_s;
+    const uint32_t logical = (blockIdx.x % 8u) * A.tiles.per_xcd + blockIdx.x / 8u;
+    if (logical >= A.tiles.ntiles) return;   // block-uniform
+    int32_t px, py;
+    const bool inview = tile_pixel(A.tiles, px, py);
+    const size_t i = inview ? (size_t)py * A.view.width + px : 0;
+    const uint32_t m = inview ? A.mat[i] : 0u;
+    const bool valid = m != 0;
+    f3 pp = mk3(0.0f, 0.0f, 0.0f);
+    if (valid) pp = load3(A.pos + 3 * i);
+    const uint32_t n = tile_light_list(A, valid, mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f), list_s, box_s,
+                                       wcount_s, &bad_s);
+    if (!inview) return;
+    if (!valid) {
+        if (A.fetches) A.fetches[i] = 0;
+        return;
+    }
+    LightPix P;
+    light_pixel_setup(A, i, P);
     float acc[3] = {A.rgb[3 * i + 0], A.rgb[3 * i + 1], A.rgb[3 * i + 2]};
     uint32_t nf = 0;
     for (uint32_t k = 0; k < n; ++k) {
         const vx_light& L = A.lights[__builtin_amdgcn_readfirstlane(list_s[k])];
-        const f3 ld = mk3(L.position[0] - wpos.x, L.position[1] - wpos.y, L.position[2] - wpos.z);
-        const float dist = sqrtf(dot3(ld, ld));
-        if (dist > L.range) continue;
-        const bool spot = L.type == VX_LIGHT_SPOT;
-        f3 wd;
+        f3 ld;
+        float dist;
+        if (!light_in_range(L, P.wpos, ld, dist)) continue;
+        f3 o, wdn;
         float hit_dist;
-        if (spot) {
-            const f3 nl = normalize3(ld);
-            wd = mk3(nl.x * 10.0f, nl.y * 10.0f, nl.z * 10.0f);
-            hit_dist = dist * 10.0f;
-        } else {
-            wd = ld;
-            hit_dist = dist * 10.5f;
-        }
-        const f3 wdn = normalize3(mk3(wd.x * (1.0f - 0.5f) + rv.x * 0.5f, wd.y * (1.0f - 0.5f) + rv.y * 0.5f,
-                                      wd.z * (1.0f - 0.5f) + rv.z * 0.5f));
-        f3 o;
-        {
-            float c0 = wpos.x * 10.0f, c1 = wpos.y * 10.0f, c2 = wpos.z * 10.0f;
-            c0 = c0 + wdn.x * (r0[3] * 1.0f); c1 = c1 + wdn.y * (r0[3] * 1.0f); c2 = c2 + wdn.z * (r0[3] * 1.0f);
-            c0 = c0 + rv.x * 2.5f; c1 = c1 + rv.y * 2.5f; c2 = c2 + rv.z * 2.5f;
-            o = mk3(c0 + N.x * 0.5f, c1 + N.y * 0.5f, c2 + N.z * 0.5f);
-        }
+        light_ray(L, P, ld, dist, o, wdn, hit_dist);
+        const bool spot = L.type == VX_LIGHT_SPOT;
         int32_t hit;
         const float d = march_dist(V, spot ? A.params.spot_march : A.params.point_march, o, wdn, hit_dist, nf, hit);
-        const float shadow = (d < hit_dist) ? 0.0f : 1.0f;
-        float att = gclamp(L.range - dist, 0.0f, 1.0f) / pow_vx(dist, L.attenuation);
-        if (spot) {
-            const f3 nl = normalize3(ld);
-            const f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
-            const float ad = gmax(dot3(nl, dir) - (1.0f - L.angle), 0.0f) / L.angle;
-            att = (pow_vx(ad, L.angle_attenuation) * gclamp(L.range - dist, 0.0f, 1.0f)) / pow_vx(dist, L.attenuation);
-        }
-        const float ndl = gmax(dot3(N, ld), 0.0f);
+        float term[3];
+        light_term(L, P, ld, dist, (d < hit_dist) ? 0.0f : 1.0f, term);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float rad = (L.color[c] * att) * shadow;
-            acc[c] = acc[c] + (((kD * e[c]) / 3.14159265359f) * rad) * ndl;
-        }
+        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + term[c];
     }
     A.rgb[3 * i + 0] = acc[0];
     A.rgb[3 * i + 1] = acc[1];
     A.rgb[3 * i + 2] = acc[2];
+    if (A.fetches) A.fetches[i] = nf;
+}
+
+// ---------------------------------------------------------------------------
+// The light pass as a ray list: a pixel's (pixel, light) shadow rays are
+// listed, marched one per lane, and their terms folded per pixel in light
+// order. Pass 1 counts each pixel's in-range lights and gives each 16x16
+// tile a contiguous range of the list (one atomic per tile); pass 2 writes
+// the records; pass 3 marches every record and stores its term; pass 4 adds
+// a pixel's terms to its light buffer value in list order, which is draw
+// order. The arithmetic is k_lights' (the same device functions).
+// ---------------------------------------------------------------------------
+template <bool WRITE>
+VXD void light_list_tile(const LightArgs& A, const LightListArgs& B) {
+    __shared__ uint16_t list_s[kMaxLights];
+    __shared__ float box_s[4][6];
+    __shared__ uint32_t wcount_s[4];
+    __shared__ uint32_t bad_s;
+    __shared__ uint32_t wsum_s[4];
+    __shared__ uint32_t tile_base_s;
+    const uint32_t logical = (blockIdx.x % 8u) * A.tiles.per_xcd + blockIdx.x / 8u;
+    if (logical >= A.tiles.ntiles) return;   // block-uniform
+    int32_t px, py;
+    const bool inview = tile_pixel(A.tiles, px, py);
+    const size_t i = inview ? (size_t)py * A.view.width + px : 0;
+    const bool valid = inview && A.mat[i] != 0;
+    f3 wpos = mk3(0.0f, 0.0f, 0.0f);
+    if (valid) {
+        const f3 pp = load3(A.pos + 3 * i);
+        wpos = mk3(pp.x * 0.1f, pp.y * 0.1f, pp.z * 0.1f);
+    }
+    const uint32_t n = tile_light_list(A, valid, wpos, list_s, box_s, wcount_s, &bad_s);
+    if (!WRITE) {
+        uint32_t c = 0;
+        if (valid)
+            for (uint32_t k = 0; k < n; ++k) {
+                f3 ld;
+                float dist;
+                c += light_in_range(A.lights[__builtin_amdgcn_readfirstlane(list_s[k])], wpos, ld, dist) ? 1u : 0u;
+            }
+        // the tile's exclusive scan: in-wave, then over the four waves
+        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        uint32_t x = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)lane >= off) x += y;
+        }
+        if (lane == 63u) wsum_s[wave] = x;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (uint32_t w = 0; w < wave; ++w) wbase += wsum_s[w];
+        if (threadIdx.x == 0)
+            tile_base_s = atomicAdd(B.total, (wsum_s[0] + wsum_s[1]) + (wsum_s[2] + wsum_s[3]));
+        __syncthreads();
+        if (inview) {
+            B.first[i] = tile_base_s + wbase + (x - c);
+            B.count[i] = c;
+        }
+    } else {
+        if (!valid) return;
+        uint32_t j = B.first[i];
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t l = list_s[k];
+            f3 ld;
+            float dist;
+            if (!light_in_range(A.lights[__builtin_amdgcn_readfirstlane(l)], wpos, ld, dist)) continue;
+            B.rec[j].pixel = (uint32_t)i;
+            B.rec[j].light = l;
+            ++j;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_light_count(LightArgs A, LightListArgs B) { light_list_tile<false>(A, B); }
+__global__ void __launch_bounds__(256) k_light_records(LightArgs A, LightListArgs B) { light_list_tile<true>(A, B); }
+
+// One shadow ray per thread over the list (grid-stride to the device total).
+__global__ void __launch_bounds__(256) k_light_march(OccView V, LightArgs A, LightListArgs B) {
+    const uint32_t total = *B.total;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
+        const LightRec r = B.rec[j];
+        LightPix P;
+        light_pixel_setup(A, r.pixel, P);
+        const vx_light& L = A.lights[r.light];
+        f3 ld;
+        float dist;
+        (void)light_in_range(L, P.wpos, ld, dist);
+        f3 o, wdn;
+        float hit_dist;
+        light_ray(L, P, ld, dist, o, wdn, hit_dist);
+        uint32_t nf = 0;
+        int32_t hit;
+        const float d =
+            march_dist(V, L.type == VX_LIGHT_SPOT ? A.params.spot_march : A.params.point_march, o, wdn, hit_dist, nf, hit);
+        float term[3];
+        light_term(L, P, ld, dist, (d < hit_dist) ? 0.0f : 1.0f, term);
+        B.term[3 * (size_t)j + 0] = term[0];
+        B.term[3 * (size_t)j + 1] = term[1];
+        B.term[3 * (size_t)j + 2] = term[2];
+        if (B.rfetch) B.rfetch[j] = (uint16_t)nf;
+    }
+}
+
+// A pixel's terms added to its light buffer value in list (draw) order.
+__global__ void __launch_bounds__(256) k_light_fold(LightArgs A, LightListArgs B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)(A.view.width * A.view.height)) return;
+    if (A.mat[i] == 0) {
+        if (A.fetches) A.fetches[i] = 0;
+        return;
+    }
+    const uint32_t j0 = B.first[i], n = B.count[i];
+    float acc[3] = {A.rgb[3 * (size_t)i + 0], A.rgb[3 * (size_t)i + 1], A.rgb[3 * (size_t)i + 2]};
+    uint32_t nf = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const float* t = B.term + 3 * (size_t)(j0 + k);
+        acc[0] = acc[0] + t[0];
+        acc[1] = acc[1] + t[1];
+        acc[2] = acc[2] + t[2];
+        if (A.fetches) nf += B.rfetch[j0 + k];
+    }
+    A.rgb[3 * (size_t)i + 0] = acc[0];
+    A.rgb[3 * (size_t)i + 1] = acc[1];
+    A.rgb[3 * (size_t)i + 2] = acc[2];
     if (A.fetches) A.fetches[i] = nf;
 }
 
@@ -1965,6 +2201,224 @@ __global__ void __launch_bounds__(256) k_light_taa(TaaArgs A) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) o[k] = cur[k] * (1.0f - t) + last[k] * t;
     o[3] = gclamp(lvar * 0.7f, 0.0f, 1.0f);
+}
+
+// ===========================================================================
+// Reflection pass: LightReflection.frag:60-139 (LightReflectionPipeline::Use,
+// LightReflectionPipeline.h:34), one thread per pixel, world space. The
+// reflection ray's march, its reprojection into the light buffer with the
+// +-0.001 linear-depth window against the depth attachment, the sky on a
+// miss, and (ambient * F * (1 - roughness), F) with F the Schlick-roughness
+// Fresnel of albedo 1, times 5. Light and depth are sampled with the TAA
+// pass's sampler (bilinear, 8-bit sub-texel weights, opaque-black border).
+// ===========================================================================
+namespace refl {
+// getSkyColor (LightAmbient.frag:128-131).
+VXD f3 sky_procedural(f3 e) {
+    const float ey = (gmax(e.y, 0.0f) * 0.8f + 0.2f) * 0.8f;
+    const float a = 1.0f - ey;
+    return mk3((a * a) * 1.1f, a * 1.1f, (0.6f + a * 0.4f) * 1.1f);
+}
+
+// texture(samplerCube, d): Vulkan face selection (major axis x, then y, then
+// z on ties), bilinear within the face with the texel position clamped to
+// it. Faces +X, -X, +Y, -Y, +Z, -Z, size x size RGBA.
+VXD f3 cube_sample(const float* faces, int32_t size, f3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int32_t face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x >= 0.0f) { face = 0; sc = -d.z; tc = -d.y; }
+        else { face = 1; sc = d.z; tc = -d.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d.y >= 0.0f) { face = 2; sc = d.x; tc = d.z; }
+        else { face = 3; sc = d.x; tc = -d.z; }
+    } else {
+        ma = az;
+        if (d.z >= 0.0f) { face = 4; sc = d.x; tc = -d.y; }
+        else { face = 5; sc = -d.x; tc = -d.y; }
+    }
+    const float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
+    const float fs = (float)size;
+    const float x = gclamp(s * fs - 0.5f, 0.0f, fs - 1.0f), y = gclamp(t * fs - 0.5f, 0.0f, fs - 1.0f);
+    int32_t x0, y0;
+    float fx, fy;
+    taa::weight(x, size, x0, fx);
+    taa::weight(y, size, y0, fy);
+    const int32_t x1 = x0 + 1 < size ? x0 + 1 : size - 1, y1 = y0 + 1 < size ? y0 + 1 : size - 1;
+    const float4* F = reinterpret_cast<const float4*>(faces) + (size_t)face * size * size;
+    const float4 a = F[(size_t)y0 * size + x0], b = F[(size_t)y0 * size + x1];
+    const float4 c = F[(size_t)y1 * size + x0], e = F[(size_t)y1 * size + x1];
+    return mk3(taa::bl(a.x, b.x, c.x, e.x, fx, fy), taa::bl(a.y, b.y, c.y, e.y, fx, fy),
+               taa::bl(a.z, b.z, c.z, e.z, fx, fy));
+}
+
+// Bilinear sample of an n-channel float image (border: 0, alpha 1).
+template <int N>
+VXD void sample(const float* img, int32_t w, int32_t h, float u, float v, float* out) {
+    int32_t x0, y0;
+    float fx, fy;
+    taa::weight(u * (float)w - 0.5f, w, x0, fx);
+    taa::weight(v * (float)h - 0.5f, h, y0, fy);
+    float t[4][N];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t x = x0 + (k & 1), y = y0 + (k >> 1);
+        const bool in = !(x < 0 || y < 0 || x >= w || y >= h);
+#pragma unroll
+        for (int c = 0; c < N; ++c) t[k][c] = in ? img[((size_t)y * w + x) * N + c] : (c == 3 ? 1.0f : 0.0f);
+    }
+#pragma unroll
+    for (int c = 0; c < N; ++c) out[c] = taa::bl(t[0][c], t[1][c], t[2][c], t[3][c], fx, fy);
+}
+}  // namespace refl
+
+__global__ void __launch_bounds__(256) k_reflection(OccView V, ReflectionArgs A) {
+    int32_t px, py;
+    if (!tile_pixel(A.tiles, px, py)) return;
+    const vx_view& vw = A.view;
+    const int32_t W = vw.width, H = vw.height;
+    const size_t i = (size_t)py * W + px;
+    const uint32_t m = A.mat[i];
+    const f3 pp = load3(A.pos + 3 * i), n = load3(A.nrm + 3 * i);
+    const float depth = A.depth[i];
+    const float rough = A.palette[6 * m + 3], metal = A.palette[6 * m + 4];
+    const float f0 = 0.04f * (1.0f - metal) + 1.0f * metal;
+    const f3 cam = mk3(vw.cam_pos[0] * 10.0f, vw.cam_pos[1] * 10.0f, vw.cam_pos[2] * 10.0f);
+    const f3 Vd = normalize3(mk3(cam.x - pp.x, cam.y - pp.y, cam.z - pp.z));
+    const float ndv = gmax(dot3(n, Vd), 0.0f);
+    const float x = gmax(1.0f - ndv, 0.0f);
+    const float s5 = (((x * x) * x) * x) * x;
+    const float F = (f0 + (gmax(1.0f - rough, f0) - f0) * s5) * 5.0f + 0.0f;
+    f3 amb = mk3(0.0f, 0.0f, 0.0f);
+    float tt = __uint_as_float(0x7fc00000u);
+    if (depth < 0.999f) {
+        // the ray (LightReflection.frag:79-98)
+        const uint32_t pk = pixkey(A.params.seed, A.params.frame, (uint32_t)i);
+        float nz[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nz[k] = rand01(pk, (uint32_t)k);
+        const f3 I = mk3(-Vd.x, -Vd.y, -Vd.z);
+        const float dni = dot3(n, I);
+        f3 wd = normalize3(mk3(I.x - (2.0f * dni) * n.x, I.y - (2.0f * dni) * n.y, I.z - (2.0f * dni) * n.z));
+        f3 rv = cosine_hemisphere(nz[0], nz[1]);
+        rv.z = rv.z * gsign(nz[2] - 0.5f);
+        const float a = rough * 0.1f;
+        wd = mk3(wd.x * (1.0f - a) + rv.x * a, wd.y * (1.0f - a) + rv.y * a, wd.z * (1.0f - a) + rv.z * a);
+        const f3 wcp = mk3(pp.x + n.x * nz[3], pp.y + n.y * nz[3], pp.z + n.z * nz[3]);
+        const float sc = 1.0f + nz[3] * 0.5f;
+        wd = mk3(wd.x * sc, wd.y * sc, wd.z * sc);
+        uint32_t nf = 0;
+        int32_t hit;
+        const float t = march_dist(V, A.params.march, mk3(wcp.x + n.x, wcp.y + n.y, wcp.z + n.z), wd, A.params.maxt,
+                                   nf, hit);
+        tt = t;
+        // reprojection (LightReflection.frag:115-133)
+        float h4[4] = {(wcp.x + wd.x * t) * 0.1f, (wcp.y + wd.y * t) * 0.1f, (wcp.z + wd.z * t) * 0.1f, 1.0f};
+        float pv[4], pc[4];
+        mat_vec(vw.view, h4, pv);
+        pv[3] = 1.0f;
+        mat_vec(vw.proj, pv, pc);
+        const float lin = (pc[3] - 0.1f) / (4096.0f - 0.1f);
+        const float u = ((pc[0] / pc[3]) * 1.0f) * 0.5f + 0.5f;
+        const float v = ((pc[1] / pc[3]) * -1.0f) * 0.5f + 0.5f;
+        float ds;
+        refl::sample<1>(A.depth, W, H, u, v, &ds);
+        if (t == A.params.maxt) {
+            amb = A.sky ? refl::cube_sample(A.sky, A.sky_size, wd) : refl::sky_procedural(wd);
+        } else if (lin > ds - 0.001f && lin < ds + 0.001f) {
+            float l4[4];
+            refl::sample<4>(A.light, W, H, u, v, l4);
+            amb = mk3(l4[0], l4[1], l4[2]);
+        }
+    }
+    float4 o;
+    o.x = (amb.x * F) * (1.0f - rough);
+    o.y = (amb.y * F) * (1.0f - rough);
+    o.z = (amb.z * F) * (1.0f - rough);
+    o.w = F;
+    reinterpret_cast<float4*>(A.out)[i] = o;
+    if (A.t) A.t[i] = tt;
+}
+
+// Screen-space occlusion, calculateOcclusion (LightAmbient.frag:89-109) with
+// screenspaceOcclusion (:59-79), one thread per pixel: the view-space
+// position rebuilt from the depth attachment and FarVec, four samples within
+// OCCLUSION_SIZE = 0.2 world units in the normal's cosine hemisphere (view
+// space), each occluded where the depth attachment at its projection lies
+// within OCCLUSION_TICKNESS / FAR in front of it, weighted by
+// smoothstep(maxDepth, minDepth, depth) (edges reversed, evaluated as
+// written) times its distance; 1 - 3 * mean, clamped. The noise is the
+// pixel's counter hash under its own key. 1 for sky pixels.
+__global__ void __launch_bounds__(256) k_ssao(SsaoArgs A) {
+    const vx_view& vw = A.view;
+    const int32_t W = vw.width, H = vw.height;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)(W * H)) return;
+    const float depth = A.depth[i];
+    if (A.mat[i] == 0 || !(depth < 0.999f)) {
+        A.out[i] = 1.0f;
+        return;
+    }
+    const int32_t px = (int32_t)(i % (uint32_t)W), py = (int32_t)(i / (uint32_t)W);
+    float n4[4] = {A.nrm[3 * (size_t)i], A.nrm[3 * (size_t)i + 1], A.nrm[3 * (size_t)i + 2], 0.0f}, N4[4];
+    mat_vec(vw.view, n4, N4);
+    const f3 N = mk3(N4[0], N4[1], N4[2]);
+    const f3 T = normalize3(fabsf(N.z) > 0.5f ? mk3(0.0f, -N.z, N.y) : mk3(-N.y, N.x, 0.0f));
+    const f3 B = normalize3(mk3(N.y * T.z - N.z * T.y, N.z * T.x - N.x * T.z, N.x * T.y - N.y * T.x));
+    const float u = ((float)px + 0.5f) / vw.res[0], v = ((float)py + 0.5f) / vw.res[1];
+    float c4[4] = {u * 2.0f - 1.0f, 1.0f - v * 2.0f, 1.0f, 1.0f}, f4[4];
+    mat_vec(vw.inv_proj, c4, f4);
+    const float fx = f4[0] / f4[3], fy = f4[1] / f4[3], fz = f4[2] / f4[3];
+    const float sc = depth * (1.0f + 1.0f / 4096.0f) + 0.1f / 4096.0f;
+    const f3 pos = mk3(fx * sc, fy * sc, fz * sc);
+    const float size = 0.2f * (1.0f + depth * 0.0f);
+    const uint32_t pk = pixkey(A.seed ^ 0x632be5abu, vw.frame, i);
+    float occ = 0.0f;
+#pragma unroll 1
+    for (int s = 0; s < 4; ++s) {
+        const float nx = rand01(pk, (uint32_t)(4 * s)), ny = rand01(pk, (uint32_t)(4 * s + 1));
+        const float dist = rand01(pk, (uint32_t)(4 * s + 2));
+        const f3 rv = cosine_hemisphere(nx, ny);
+        const f3 dn = normalize3(mk3((T.x * rv.x + B.x * rv.y) + N.x * rv.z, (T.y * rv.x + B.y * rv.y) + N.y * rv.z,
+                                     (T.z * rv.x + B.z * rv.y) + N.z * rv.z));
+        float mid[4] = {pos.x + (dn.x * size) * dist, pos.y + (dn.y * size) * dist, pos.z + (dn.z * size) * dist, 1.0f};
+        float mp[4];
+        mat_vec(vw.proj, mid, mp);
+        const float sd = (mp[3] - 0.1f) / (4096.0f - 0.1f);
+        const float su = ((mp[0] / mp[3]) * 1.0f) * 0.5f + 0.5f;
+        const float sv = ((mp[1] / mp[3]) * -1.0f) * 0.5f + 0.5f;
+        float mn;
+        refl::sample<1>(A.depth, W, H, su, sv, &mn);
+        const float mx = mn + 0.2f / 4096.0f;
+        float o = 0.0f;
+        if (gclamp(su, 0.0f, 1.0f) == su && gclamp(sv, 0.0f, 1.0f) == sv && mn < sd && sd < mx) {
+            const float tt = gclamp((sd - mx) / (mn - mx), 0.0f, 1.0f);
+            o = ((tt * tt) * (3.0f - 2.0f * tt)) * dist;
+        }
+        occ = occ + o;
+    }
+    occ = occ / 4.0f;
+    occ = occ * 3.0f;
+    A.out[i] = gclamp(1.0f - occ, 0.0f, 1.0f);
+}
+
+// D24 depth of a G-buffer's hits: the geometry pass's fragment depth// D24 depth of a G-buffer's hits: the geometry pass's fragment depth
+// (1 - 1e-7 rid)(w - NEAR)/(FAR - NEAR) of the world hit, quantized to 24
+// bits and read back as q / (2^24 - 1); 1 where nothing was hit.
+__global__ void __launch_bounds__(256) k_gbuf_depth(DepthArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.npx) return;
+    uint32_t q = 16777215u;
+    if (A.mat[i]) {
+        float h4[4] = {A.pos[3 * (size_t)i] * 0.1f, A.pos[3 * (size_t)i + 1] * 0.1f, A.pos[3 * (size_t)i + 2] * 0.1f,
+                       1.0f}, c[4];
+        mat_vec(A.pv, h4, c);
+        q = depth_d24((A.rid_scale * (c[3] - 0.1f)) / (4096.0f - 0.1f));
+    }
+    A.depth[i] = (float)q / 16777215.0f;
 }
 
 // ===========================================================================

@@ -85,6 +85,18 @@ def scene_view(dims, width, height, frame=0, variant=0):
     return make_view(eye, tgt, width, height, frame=frame)
 
 
+def city_view(dims, width, height, frame=0, variant=0):
+    """Camera inside a low city (config L): 0.293 ny above one corner of the
+    ground, looking down the diagonal at the ground a quarter of the way to
+    the centre, so that nearly every pixel sees a block. variant picks the
+    corner."""
+    nx, ny, nz = dims
+    cx, cz = [(0.02, 0.02), (0.98, 0.02), (0.98, 0.98), (0.02, 0.98)][variant % 4]
+    eye = np.array([cx * nx, 0.293 * ny, cz * nz]) / 10.0
+    tgt = np.array([(cx + (0.5 - cx) * 0.5) * nx, 0.0, (cz + (0.5 - cz) * 0.5) * nz]) / 10.0
+    return make_view(eye, tgt, width, height, frame=frame)
+
+
 def sun_dir():
     """SUN_DIR = normalize(vec3(0.3,0.4,0.5)) (LightAmbient.frag:15), in fp32 as
     v / sqrt(dot(v, v))."""
