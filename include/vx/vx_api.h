@@ -278,6 +278,21 @@ vx_status vx_primary_instances(vx_ctx* ctx, const vx_vox_cmd* cmds, uint32_t n, 
 vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
                    const float* palette_dev, const vx_trace_params* params, const vx_trace_out* out,
                    void* stream);
+/* The same trace as vx_trace, with no allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * ray-parallel path's scratch (per-ray results, listed rays, fetch counts,
+ * queue counters) is a file-scope device array of the library, one per GPU,
+ * of 2^22 ray slots (about 40 MiB), held until the process ends; the view is
+ * traced in consecutive ranges of 8x8 tiles whose rays fit it (at most
+ * 2^22 / (64 * rays per pixel) tiles per range), one range after another on
+ * `stream`. Every call on a GPU shares that GPU's scratch: calls that overlap
+ * on one GPU (other streams, other threads, other contexts) race and must be
+ * serialized by the caller. No kernel timing (vx_ctx_profile) is recorded.
+ * VX_ERR_RANGE if one tile's rays exceed the scratch. */
+vx_status vx_trace_noalloc(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* gbuf,
+                           const float* palette_dev, const vx_trace_params* params, const vx_trace_out* out,
+                           void* stream);
 /* vx_trace (all rays through the DDA marcher) that also writes every ray's
  * hit record: hits_dev[pixel * nrays + r] for ray r of a pixel (r in the
  * order shadow, AO, spec; nrays = n_shadow + n_ao + n_spec), all zero for sky

@@ -163,6 +163,8 @@ SIGNATURES = {
     "vx_primary": ([P, P, C.POINTER(View), C.POINTER(GBuf), P], C.c_int32),
     "vx_trace": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut), P],
                  C.c_int32),
+    "vx_trace_noalloc": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut),
+                          P], C.c_int32),
     "vx_trace_hits": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut),
                        P, P], C.c_int32),
     "vx_trace_rays": ([P, P, P, C.c_uint32, C.c_int32, P, P], C.c_int32),

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -99,6 +100,37 @@ static void march_buffers_free(MarchBuffers& m) {
     std::memset(&m, 0, sizeof(m));
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of vx_trace_noalloc: one file-scope device array per GPU, sized for
+// kNoAllocSlots ray slots (result, listed-ray and fetch-count arrays) plus the
+// queue counters. Host code never names the array in a launch: its device
+// address is taken once per device with hipGetSymbolAddress (vx_ctx_create)
+// and kept in g_noalloc_ptr.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocSlots = size_t(1) << 22;
+constexpr size_t kNoAllocCounterBytes = 4096;
+constexpr size_t kNoAllocBytes =
+    kNoAllocCounterBytes + kNoAllocSlots * (sizeof(float) + sizeof(uint32_t) + sizeof(uint16_t));
+static_assert(kCounterWords * sizeof(uint32_t) <= kNoAllocCounterBytes, "queue counters fit their region");
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static uint8_t* g_noalloc_ptr[kMaxDevices];
+
+// Device address of this device's scratch (the device must be current).
+static hipError_t noalloc_scratch(int device, uint8_t** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[device] = static_cast<uint8_t*>(p);
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
+
 // Persistent-grid size of one march kernel, cached per context: the device's
 // CU count and the kernel's occupancy do not change between launches.
 struct GridCache {
@@ -109,6 +141,7 @@ struct GridCache {
 struct vx_ctx {
     int32_t device;
     GridCache gcache;
+    uint8_t* noalloc;        // this device's vx_trace_noalloc scratch (device address)
     long long* grad_acc;     // 256*6+6 fixed-point gradient accumulator
     ShadeArgs* shade_items;  // batched adjoint: per-view arguments (device)
     uint32_t* shade_block0;  // and each view's first block (n + 1 entries)
@@ -265,7 +298,10 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
         return fail(VX_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
     }
     VX_REQUIRE(device >= 0 && device < n, "device %d out of range (0..%d)", device, n - 1);
+    VX_REQUIRE(device < kMaxDevices, "device %d beyond the library's %d-device limit", device, kMaxDevices);
     VX_HIP(hipSetDevice(device));
+    uint8_t* noalloc = nullptr;
+    VX_HIP(noalloc_scratch(device, &noalloc));
     int cus = 0;
     VX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     vx_ctx* c = new (std::nothrow) vx_ctx();
@@ -273,6 +309,7 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     c->device = device;
     std::memset(&c->gcache, 0, sizeof(c->gcache));
     c->gcache.cus = cus;
+    c->noalloc = noalloc;
     c->grad_acc = nullptr;
     c->shade_items = nullptr;
     c->shade_block0 = nullptr;
@@ -1337,6 +1374,30 @@ static vx_status launch_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tra
     return enqueue_trace_march(ctx, vol, T, 0, ntiles, M, stream, e0, e1, hits);
 }
 
+// vx_trace_noalloc's march: the view's tiles in consecutive ranges whose ray
+// slots fit the device's scratch, one after another on the stream (a range
+// reuses the scratch once the previous range's resolve has read it).
+static vx_status launch_trace_march_noalloc(vx_ctx* ctx, const vx_volume* vol, const TraceArgs& T,
+                                            hipStream_t stream) {
+    const uint32_t ntiles = tile_geom(ctx->tuning, T.view.width, T.view.height).ntiles;
+    const size_t per_tile = (size_t)(T.params.n_shadow + T.params.n_ao + T.params.n_spec) * 64;
+    if (per_tile > kNoAllocSlots)
+        return fail(VX_ERR_RANGE, "an 8x8 tile needs %zu ray slots; the no-allocation scratch holds %zu", per_tile,
+                    kNoAllocSlots);
+    const uint32_t chunk = (uint32_t)std::max<size_t>(kNoAllocSlots / std::max<size_t>(per_tile, 1), 1);
+    uint8_t* base = ctx->noalloc;
+    MarchScratch M;
+    M.counters = reinterpret_cast<uint32_t*>(base);
+    M.res = reinterpret_cast<float*>(base + kNoAllocCounterBytes);
+    M.deferred = reinterpret_cast<uint32_t*>(base + kNoAllocCounterBytes + kNoAllocSlots * sizeof(float));
+    M.fres = reinterpret_cast<uint16_t*>(base + kNoAllocCounterBytes + kNoAllocSlots * (sizeof(float) + sizeof(uint32_t)));
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += chunk) {
+        const uint32_t n = std::min(chunk, ntiles - t0);
+        if (vx_status st = enqueue_trace_march(ctx, vol, T, t0, n, M, stream, nullptr, nullptr)) return st;
+    }
+    return VX_OK;
+}
+
 static vx_status check_params(const vx_trace_params* p) {
     VX_REQUIRE(p, "params is NULL");
     auto okm = [](int32_t m) { return m == VX_MARCH_DDA || m == VX_MARCH_SPARSE || m == VX_MARCH_SUPERSPARSE; };
@@ -1350,7 +1411,7 @@ static vx_status check_params(const vx_trace_params* p) {
 
 static vx_status trace_impl(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
                             const float* palette, const vx_trace_params* params, const vx_trace_out* out,
-                            void* stream, vx_trace_hit* hits = nullptr) {
+                            void* stream, bool noalloc, vx_trace_hit* hits = nullptr) {
     VX_REQUIRE(ctx && vol && g && out && palette, "ctx/volume/gbuf/out/palette is NULL");
     if (vx_status st = check_view(view)) return st;
     if (vx_status st = check_params(params)) return st;
@@ -1380,7 +1441,7 @@ static vx_status trace_impl(vx_ctx* ctx, const vx_volume* vol, const vx_view* vi
         return launch_trace_march(ctx, vol, A, S(stream), hits);
     }
     if (all_dda && ctx->tuning.kernel != 1)
-        return launch_trace_march(ctx, vol, A, S(stream));
+        return noalloc ? launch_trace_march_noalloc(ctx, vol, A, S(stream)) : launch_trace_march(ctx, vol, A, S(stream));
     hipLaunchKernelGGL(k_trace, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
     VX_LAUNCHED("k_trace");
     return VX_OK;
@@ -1389,14 +1450,20 @@ static vx_status trace_impl(vx_ctx* ctx, const vx_volume* vol, const vx_view* vi
 extern "C" vx_status vx_trace(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
                               const float* palette, const vx_trace_params* params, const vx_trace_out* out,
                               void* stream) {
-    return trace_impl(ctx, vol, view, g, palette, params, out, stream);
+    return trace_impl(ctx, vol, view, g, palette, params, out, stream, false);
 }
 
 extern "C" vx_status vx_trace_hits(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
                                    const float* palette, const vx_trace_params* params, const vx_trace_out* out,
                                    vx_trace_hit* hits, void* stream) {
     VX_REQUIRE(hits, "hits is NULL");
-    return trace_impl(ctx, vol, view, g, palette, params, out, stream, hits);
+    return trace_impl(ctx, vol, view, g, palette, params, out, stream, false, hits);
+}
+
+extern "C" vx_status vx_trace_noalloc(vx_ctx* ctx, const vx_volume* vol, const vx_view* view, const vx_gbuf* g,
+                                      const float* palette, const vx_trace_params* params, const vx_trace_out* out,
+                                      void* stream) {
+    return trace_impl(ctx, vol, view, g, palette, params, out, stream, true);
 }
 
 extern "C" vx_status vx_trace_rays(vx_ctx* ctx, const vx_volume* vol, const float* rays, uint32_t n, int32_t march,
