@@ -174,6 +174,14 @@ typedef struct {
 /* ---------------------------------------------------------------- context */
 int32_t vx_api_version(void);
 const char* vx_last_error(void);
+/* Failure injection for tests of error handling (no counterpart in the
+ * reference): the `after`-th device allocation (VX_FAIL_ALLOC) or kernel
+ * launch check (VX_FAIL_LAUNCH) from now on, counting from 0, fails once
+ * (the call returns VX_ERR_OUT_OF_MEMORY / VX_ERR_DEVICE and releases what it
+ * allocated); -1 injects nothing. The environment variables VX_FAIL_ALLOC=n
+ * and VX_FAIL_LAUNCH=n set the same counters when the library loads. */
+enum { VX_FAIL_ALLOC = 0, VX_FAIL_LAUNCH = 1 };
+vx_status vx_debug_inject_failure(int32_t kind, int64_t after);
 /* Replaces Graphics module start-up (Graphics.cpp) for this path: selects the
  * HIP device and allocates per-device state. */
 vx_status vx_ctx_create(int32_t device, vx_ctx** out);
@@ -322,6 +330,15 @@ typedef struct {
 vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette_dev,
                           const vx_trace_params* params, vx_batch** out);
 vx_status vx_batch_launch(vx_batch* batch, void* stream);
+/* Replaces the batch's items (any number, any view sizes) keeping its palette
+ * and ray set-up: waits for the device, re-plans the tile queue, re-uploads
+ * the descriptor table, grows the scratch when needed and captures the launch
+ * again (the launch's shape lives in the graph). Arguments are checked
+ * first; if anything fails after that, the batch reports VX_ERR_STATE until a
+ * later update succeeds. */
+vx_status vx_batch_update(vx_batch* batch, const vx_batch_item* items, uint32_t n);
+/* The batch's item count and how many times its launch has been captured. */
+vx_status vx_batch_info(const vx_batch* batch, uint32_t* nitems, uint32_t* captures);
 /* The same work as vx_batch_launch, launched directly instead of through the graph. */
 vx_status vx_batch_launch_direct(vx_batch* batch, void* stream);
 vx_status vx_batch_destroy(vx_batch* batch);

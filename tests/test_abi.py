@@ -52,3 +52,17 @@ def test_no_device_is_an_error_not_a_fallback():
     out = C.c_void_p()
     assert L.vx_ctx_create(0, C.byref(out)) == _lib.VX_ERR_DEVICE
     assert not out.value
+
+
+def test_failure_injection_arguments():
+    """vx_debug_inject_failure checks its kind and count; -1 disarms."""
+    from voxelengine_amd import _lib
+    L = _lib.lib()
+    assert L.vx_debug_inject_failure(7, 0) == _lib.VX_ERR_INVALID_ARGUMENT
+    assert L.vx_debug_inject_failure(0, -2) == _lib.VX_ERR_INVALID_ARGUMENT
+    for kind in (0, 1):
+        assert L.vx_debug_inject_failure(kind, 3) == 0
+        assert L.vx_debug_inject_failure(kind, -1) == 0
+    out = C.c_void_p()
+    assert L.vx_batch_update(None, None, 0) == _lib.VX_ERR_INVALID_ARGUMENT
+    assert L.vx_batch_info(None, None, None) == _lib.VX_ERR_INVALID_ARGUMENT

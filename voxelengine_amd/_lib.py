@@ -163,6 +163,7 @@ SIGNATURES = {
     "vx_primary": ([P, P, C.POINTER(View), C.POINTER(GBuf), P], C.c_int32),
     "vx_trace": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut), P],
                  C.c_int32),
+    "vx_debug_inject_failure": ([C.c_int32, C.c_int64], C.c_int32),
     "vx_trace_noalloc": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut),
                           P], C.c_int32),
     "vx_trace_hits": ([P, P, C.POINTER(View), C.POINTER(GBuf), P, C.POINTER(TraceParams), C.POINTER(TraceOut),
@@ -170,6 +171,8 @@ SIGNATURES = {
     "vx_trace_rays": ([P, P, P, C.c_uint32, C.c_int32, P, P], C.c_int32),
     "vx_batch_create": ([P, C.POINTER(BatchItem), C.c_uint32, P, C.POINTER(TraceParams), C.POINTER(P)], C.c_int32),
     "vx_batch_launch": ([P, P], C.c_int32),
+    "vx_batch_update": ([P, C.POINTER(BatchItem), C.c_uint32], C.c_int32),
+    "vx_batch_info": ([P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)], C.c_int32),
     "vx_batch_launch_direct": ([P, P], C.c_int32),
     "vx_batch_destroy": ([P], C.c_int32),
     "vx_shade_forward": ([P, C.POINTER(View), C.POINTER(GBuf), C.POINTER(TraceOut), P, P, C.POINTER(C.c_float),

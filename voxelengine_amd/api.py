@@ -351,16 +351,34 @@ class Batch:
 
     def __init__(self, ctx: Context, items, palette: torch.Tensor, params: dict):
         """items: list of (Volume, view dict, GBuffer, Visibility)."""
+        arr = self._items(items)
+        tp = trace_params(params)
+        h = C.c_void_p()
+        check(L.lib().vx_batch_create(ctx.h, arr, len(items), _ptr(palette), C.byref(tp), C.byref(h)))
+        self.h, self.ctx, self.items, self.palette = h, ctx, items, palette
+
+    @staticmethod
+    def _items(items):
         arr = (L.BatchItem * len(items))()
         for k, (vol, vd, g, vis) in enumerate(items):
             arr[k].volume = vol.h
             arr[k].view = view_struct(vd)
             arr[k].gbuf = g.struct()
             arr[k].out = vis.struct()
-        tp = trace_params(params)
-        h = C.c_void_p()
-        check(L.lib().vx_batch_create(ctx.h, arr, len(items), _ptr(palette), C.byref(tp), C.byref(h)))
-        self.h, self.ctx, self.items, self.palette = h, ctx, items, palette
+        return arr
+
+    def update(self, items):
+        """vx_batch_update: new items (any count and view sizes), same palette
+        and ray set-up; the launch is captured again."""
+        arr = self._items(items)
+        self.items = items
+        check(L.lib().vx_batch_update(self.h, arr, len(items)))
+
+    def info(self):
+        """(item count, graph captures so far)."""
+        n, caps = C.c_uint32(), C.c_uint32()
+        check(L.lib().vx_batch_info(self.h, C.byref(n), C.byref(caps)))
+        return n.value, caps.value
 
     def launch(self, stream=None):
         check(L.lib().vx_batch_launch(self.h, _stream(stream)))
@@ -586,3 +604,14 @@ def ssao(ctx: Context, vd: dict, gbuf: GBuffer, depth: torch.Tensor, out: torch.
     v = view_struct(vd)
     g = gbuf.struct()
     check(L.lib().vx_ssao(ctx.h, C.byref(v), C.byref(g), _ptr(depth), seed & 0xFFFFFFFF, _ptr(out), _stream(stream)))
+
+
+FAIL_ALLOC, FAIL_LAUNCH = 0, 1
+
+
+def inject_failure(kind: int, after: int):
+    """vx_debug_inject_failure: make the device allocation (FAIL_ALLOC) or
+    kernel launch (FAIL_LAUNCH) numbered `after` (0 = the next one) report
+    failure once; -1 disarms. The environment variables VX_FAIL_ALLOC and
+    VX_FAIL_LAUNCH arm the same counters when the library loads."""
+    check(L.lib().vx_debug_inject_failure(int(kind), int(after)))

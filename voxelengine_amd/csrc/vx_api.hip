@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <array>
 #include <cmath>
 #include <cstdarg>
@@ -55,6 +57,48 @@ vx_status vx_set_error(vx_status code, const char* msg) {
     return code;
 }
 
+// ---------------------------------------------------------------------------
+// Failure injection for tests of the error paths: the environment variables
+// VX_FAIL_ALLOC=n and VX_FAIL_LAUNCH=n (read when the library loads), or
+// vx_debug_inject_failure, make the n-th device allocation / kernel-launch
+// check from then on (counting from 0) fail once, with hipErrorOutOfMemory /
+// hipErrorLaunchFailure. -1 (the default) injects nothing.
+// ---------------------------------------------------------------------------
+static int64_t env_count(const char* name) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return -1;
+    char* end = nullptr;
+    const long long n = std::strtoll(v, &end, 10);
+    return (end && *end == 0 && n >= 0) ? (int64_t)n : -1;
+}
+static std::atomic<int64_t> g_fail_alloc{env_count("VX_FAIL_ALLOC")};
+static std::atomic<int64_t> g_fail_launch{env_count("VX_FAIL_LAUNCH")};
+
+// True for the one event the counter selects (it then stays at -1).
+static bool injected(std::atomic<int64_t>& c) {
+    int64_t v = c.load();
+    while (v >= 0)
+        if (c.compare_exchange_weak(v, v - 1)) return v == 0;
+    return false;
+}
+
+template <class T>
+static hipError_t dev_malloc(T** p, size_t n) {
+    if (injected(g_fail_alloc)) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, n);
+}
+
+// Status of the launches just enqueued (the last launch error), or the
+// injected launch failure.
+static hipError_t launch_status() {
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && injected(g_fail_launch)) e = hipErrorLaunchFailure;
+    return e;
+}
+
 #define VX_HIP(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
@@ -73,7 +117,7 @@ vx_status vx_set_error(vx_status code, const char* msg) {
 
 #define VX_LAUNCHED(name)                                                                     \
     do {                                                                                      \
-        hipError_t e_ = hipGetLastError();                                                    \
+        hipError_t e_ = launch_status();                                                      \
         if (e_ != hipSuccess) return fail(VX_ERR_DEVICE, "launch of %s failed: %s", name,     \
                                           hipGetErrorString(e_));                             \
     } while (0)
@@ -286,6 +330,13 @@ static vx_status check_view(const vx_view* view) {
 // context
 // ---------------------------------------------------------------------------
 extern "C" int32_t vx_api_version(void) { return VX_API_VERSION; }
+
+extern "C" vx_status vx_debug_inject_failure(int32_t kind, int64_t after) {
+    VX_REQUIRE(kind == VX_FAIL_ALLOC || kind == VX_FAIL_LAUNCH, "kind %d is not VX_FAIL_ALLOC/VX_FAIL_LAUNCH", kind);
+    VX_REQUIRE(after >= -1, "after %lld below -1", (long long)after);
+    (kind == VX_FAIL_ALLOC ? g_fail_alloc : g_fail_launch).store(after);
+    return VX_OK;
+}
 extern "C" const char* vx_last_error(void) { return g_last_error.c_str(); }
 
 extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
@@ -336,8 +387,8 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     c->rb_aabb_cap = 0;
     c->rb_ops = nullptr;
     c->rb_ops_cap = 0;
-    hipError_t me = hipMalloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
-    if (me == hipSuccess) me = hipMalloc(&c->march.counters, kCounterWords * sizeof(uint32_t));
+    hipError_t me = dev_malloc(&c->grad_acc, (256 * 6 + 6) * sizeof(long long));
+    if (me == hipSuccess) me = dev_malloc(&c->march.counters, kCounterWords * sizeof(uint32_t));
     if (me != hipSuccess) {
         (void)hipFree(c->grad_acc);
         march_buffers_free(c->march);
@@ -443,7 +494,7 @@ extern "C" vx_status vx_model_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32_
     for (int k = 0; k < 3; ++k) m->mip[k] = nullptr;
     for (int k = 0; k < 3; ++k) {
         size_t n = (size_t)(m->dims[0] >> k) * (m->dims[1] >> k) * (m->dims[2] >> k);
-        hipError_t e = hipMalloc(&m->mip[k], n);
+        hipError_t e = dev_malloc(&m->mip[k], n);
         if (e == hipSuccess) e = hipMemset(m->mip[k], 0, n);
         if (e != hipSuccess) {
             for (int j = 0; j < 3; ++j) (void)hipFree(m->mip[j]);
@@ -517,7 +568,7 @@ extern "C" vx_status vx_model_generate(vx_model* m, const vx_scene_desc* d, void
     if (vx_status st = use_device(m->ctx)) return st;
     const int32_t nx = m->dims[0], ny = m->dims[1], nz = m->dims[2];
     float* h = nullptr;
-    VX_HIP(hipMalloc(&h, (size_t)nx * nz * sizeof(float)));
+    VX_HIP(dev_malloc(&h, (size_t)nx * nz * sizeof(float)));
     hipLaunchKernelGGL(k_terrain_heights, dim3(cdiv((size_t)nx * nz, 256)), dim3(256), 0, S(stream), h, nx, ny, nz,
                        d->seed, d->terrain_fill);
     size_t nq = (size_t)nx * ny * nz / 4;
@@ -529,9 +580,12 @@ extern "C" vx_status vx_model_generate(vx_model* m, const vx_scene_desc* d, void
         hipLaunchKernelGGL(k_boxes, dim3(std::min<uint32_t>(cdiv(maxvox, 256), 64), nb), dim3(256), 0, S(stream),
                            m->mip[0], nx, ny, nz, d->seed, b0, d->box_min, d->box_max, d->boxes_grounded);
     }
-    VX_LAUNCHED("scene generation");
-    VX_HIP(hipStreamSynchronize(S(stream)));
-    VX_HIP(hipFree(h));
+    // the height map is released on every path once the kernels are done with it
+    hipError_t e = launch_status();
+    const hipError_t se = hipStreamSynchronize(S(stream));
+    (void)hipFree(h);
+    if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of scene generation failed: %s", hipGetErrorString(e));
+    VX_HIP(se);
     return VX_OK;
 }
 
@@ -580,14 +634,14 @@ extern "C" vx_status vx_volume_create(vx_ctx* ctx, int32_t sx, int32_t sy, int32
                     "indices (at most 2^32 - 1 words: with 4096 x 4096 texel slices, at most 2040 slices)",
                     sx, sy, sz, words);
     }
-    hipError_t e = hipMalloc(&v->cells, v->ncells * 8);
+    hipError_t e = dev_malloc(&v->cells, v->ncells * 8);
     if (e == hipSuccess) e = hipMemset(v->cells, 0, v->ncells * 8);
     for (int l = 0; l < nlevels && e == hipSuccess; ++l) {
         size_t nw = (size_t)v->wx[l] * v->wy[l] * v->wz[l];
-        e = hipMalloc(&v->lvl[l], nw * 8);
+        e = dev_malloc(&v->lvl[l], nw * 8);
         if (e == hipSuccess) e = hipMemset(v->lvl[l], 0, nw * 8);
     }
-    if (e == hipSuccess) e = hipMalloc(&v->grid, v->ngrid * 8);
+    if (e == hipSuccess) e = dev_malloc(&v->grid, v->ngrid * 8);
     if (e == hipSuccess) e = mgrid_full(v, nullptr);
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
@@ -649,7 +703,7 @@ extern "C" vx_status vx_volume_upload_region(vx_volume* v, const uint8_t* host, 
     if (vx_status st = use_device(v->ctx)) return st;
     size_t nb = (size_t)v->sx * v->sy * v->sz;
     uint8_t* stage = nullptr;
-    VX_HIP(hipMalloc(&stage, nb));
+    VX_HIP(dev_malloc(&stage, nb));
     hipError_t e = hipMemcpyAsync(stage, host, nb, hipMemcpyHostToDevice, S(stream));
     OccView V = occ_view(v);
     for (uint32_t i = 0; i < nregions && e == hipSuccess; ++i) {
@@ -676,7 +730,7 @@ extern "C" vx_status vx_volume_download(const vx_volume* v, uint8_t* host, size_
     VX_REQUIRE(nbytes == nb, "download buffer holds %zu bytes, volume has %zu", nbytes, nb);
     if (vx_status st = use_device(v->ctx)) return st;
     uint8_t* stage = nullptr;
-    VX_HIP(hipMalloc(&stage, nb));
+    VX_HIP(dev_malloc(&stage, nb));
     hipLaunchKernelGGL(k_cells_to_bytes, dim3(cdiv(nb, 256)), dim3(256), 0, S(stream), stage, v->cells, occ_view(v));
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(host, stage, nb, hipMemcpyDeviceToHost, S(stream));
@@ -813,7 +867,7 @@ static vx_status vx_volume_rebuild_impl(vx_volume* v, const vx_instance* inst, u
             cx->rb_aabb = nullptr;
             cx->rb_aabb_cap = 0;
             const size_t cap = std::max<size_t>(aabb_h.size(), 6 * 256);
-            VX_HIP(hipMalloc(&cx->rb_aabb, cap * sizeof(int32_t)));
+            VX_HIP(dev_malloc(&cx->rb_aabb, cap * sizeof(int32_t)));
             cx->rb_aabb_cap = cap;
         }
         if (cx->rb_ops_cap < ops.size()) {
@@ -821,7 +875,7 @@ static vx_status vx_volume_rebuild_impl(vx_volume* v, const vx_instance* inst, u
             cx->rb_ops = nullptr;
             cx->rb_ops_cap = 0;
             const size_t cap = std::max<size_t>(ops.size(), 512);
-            VX_HIP(hipMalloc(&cx->rb_ops, cap * sizeof(FootprintOp)));
+            VX_HIP(dev_malloc(&cx->rb_ops, cap * sizeof(FootprintOp)));
             cx->rb_ops_cap = cap;
         }
     }
@@ -848,7 +902,7 @@ static vx_status vx_volume_rebuild_impl(vx_volume* v, const vx_instance* inst, u
             uint32_t cnt = (uint32_t)std::min<size_t>(65535, batches[b].size() - c0);
             hipLaunchKernelGGL(k_footprint, dim3(gx, cnt), dim3(256), 0, S(stream), v->cells, V,
                                ops_d + first[b] + c0, aabb_d, thr);
-            e = hipGetLastError();
+            e = launch_status();
         }
     }
     // March grid over the conservative footprint boxes (fine voxels -> texels):
@@ -1141,8 +1195,8 @@ static vx_status vx_primary_instances_impl(vx_ctx* ctx, const vx_vox_cmd* cmds, 
             ctx->inst_rects = nullptr;
             ctx->inst_cap = 0;
             const uint32_t cap = std::max(n, 64u);
-            VX_HIP(hipMalloc(&ctx->inst, cap * sizeof(InstConst)));
-            VX_HIP(hipMalloc(&ctx->inst_rects, cap * 4 * sizeof(int32_t)));
+            VX_HIP(dev_malloc(&ctx->inst, cap * sizeof(InstConst)));
+            VX_HIP(dev_malloc(&ctx->inst_rects, cap * 4 * sizeof(int32_t)));
             ctx->inst_cap = cap;
         }
         if (ctx->tile_cap < tiles.ntiles) {
@@ -1151,8 +1205,8 @@ static vx_status vx_primary_instances_impl(vx_ctx* ctx, const vx_vox_cmd* cmds, 
             ctx->tile_lists = nullptr;
             ctx->tile_counts = nullptr;
             ctx->tile_cap = 0;
-            VX_HIP(hipMalloc(&ctx->tile_lists, (size_t)tiles.ntiles * kTileInstCap * sizeof(uint16_t)));
-            VX_HIP(hipMalloc(&ctx->tile_counts, (size_t)tiles.ntiles * sizeof(uint32_t)));
+            VX_HIP(dev_malloc(&ctx->tile_lists, (size_t)tiles.ntiles * kTileInstCap * sizeof(uint16_t)));
+            VX_HIP(dev_malloc(&ctx->tile_counts, (size_t)tiles.ntiles * sizeof(uint32_t)));
             ctx->tile_cap = tiles.ntiles;
         }
     }
@@ -1210,7 +1264,7 @@ extern "C" vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa
         (void)hipFree(ctx->taa_rec);
         ctx->taa_rec = nullptr;
         ctx->taa_cap = 0;
-        VX_HIP(hipMalloc(&ctx->taa_rec, (size_t)npx * 64));
+        VX_HIP(dev_malloc(&ctx->taa_rec, (size_t)npx * 64));
         ctx->taa_cap = npx;
     }
     A.rec = reinterpret_cast<float4*>(ctx->taa_rec);
@@ -1241,7 +1295,7 @@ static vx_status persistent_grid(vx_ctx* ctx, bool single, int block, int steps,
 // Grows the ray-parallel scratch to `slots` ray slots (and fetch counts when
 // asked); buffers in use by queued work are released after a device sync.
 static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
-    if (!m.counters) VX_HIP(hipMalloc(&m.counters, kCounterWords * sizeof(uint32_t)));
+    if (!m.counters) VX_HIP(dev_malloc(&m.counters, kCounterWords * sizeof(uint32_t)));
     const bool grow = m.slots < slots, fgrow = fetches && m.fslots < slots;
     if (!grow && !fgrow) return VX_OK;
     VX_HIP(hipDeviceSynchronize());
@@ -1251,15 +1305,15 @@ static vx_status ensure_march(MarchBuffers& m, size_t slots, bool fetches) {
         m.res = nullptr;
         m.deferred = nullptr;
         m.slots = 0;
-        VX_HIP(hipMalloc(&m.res, slots * sizeof(float)));
-        VX_HIP(hipMalloc(&m.deferred, slots * sizeof(uint32_t)));
+        VX_HIP(dev_malloc(&m.res, slots * sizeof(float)));
+        VX_HIP(dev_malloc(&m.deferred, slots * sizeof(uint32_t)));
         m.slots = slots;
     }
     if (fgrow) {
         (void)hipFree(m.fres);
         m.fres = nullptr;
         m.fslots = 0;
-        VX_HIP(hipMalloc(&m.fres, slots * sizeof(uint16_t)));
+        VX_HIP(dev_malloc(&m.fres, slots * sizeof(uint16_t)));
         m.fslots = slots;
     }
     return VX_OK;
@@ -1350,6 +1404,7 @@ static vx_status enqueue_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tr
     if (vx_status st = march_shape(ctx, true, sh)) return st;
     hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(kCounterWords, 256)), dim3(256), 0, stream, A.counters, (int)kCounterWords);
     hipError_t e = launch_march(sh.block, sh.steps, true, sh.grid, sh.rest_grid, stream, A, e0, e1);
+    if (e == hipSuccess) e = launch_status();
     if (e != hipSuccess) return fail(VX_ERR_DEVICE, "launch of the ray-parallel trace failed: %s", hipGetErrorString(e));
     return VX_OK;
 }
@@ -1488,7 +1543,7 @@ static vx_status grow(T*& p, size_t& cap, size_t n) {
     (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    VX_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    VX_HIP(dev_malloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
     cap = n;
     return VX_OK;
 }
@@ -1714,8 +1769,8 @@ static vx_status vx_shade_backward_batch_impl(vx_ctx* ctx, const vx_shade_item* 
         ctx->shade_items = nullptr;
         ctx->shade_block0 = nullptr;
         ctx->shade_cap = 0;
-        VX_HIP(hipMalloc(&ctx->shade_items, n * sizeof(ShadeArgs)));
-        VX_HIP(hipMalloc(&ctx->shade_block0, (n + 1) * sizeof(uint32_t)));
+        VX_HIP(dev_malloc(&ctx->shade_items, n * sizeof(ShadeArgs)));
+        VX_HIP(dev_malloc(&ctx->shade_block0, (n + 1) * sizeof(uint32_t)));
         ctx->shade_cap = n;
     }
     // pageable sources: the copies are complete when the calls return
@@ -1738,14 +1793,20 @@ static vx_status vx_shade_backward_batch_impl(vx_ctx* ctx, const vx_shade_item* 
 struct vx_batch {
     vx_ctx* ctx;
     uint32_t n;
+    const float* palette;
+    vx_trace_params params;
     MarchItem* items;            // device
+    size_t items_cap;
     uint16_t* tile_item;         // device: item of each global tile
+    size_t tile_cap;
     MarchBuffers march;
     MarchArgs args;
     MarchShape shape;
     hipStream_t cap;             // capture stream
     hipGraph_t graph;
     hipGraphExec_t exec;
+    uint32_t captures;           // graphs captured (1 after creation, +1 per vx_batch_update)
+    bool valid;                  // false after an update that failed part way
 };
 
 static void batch_free(vx_batch* b) {
@@ -1770,16 +1831,14 @@ static hipError_t batch_enqueue(vx_batch* b, hipStream_t s, hipEvent_t e0 = null
                         e1);
 }
 
-static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
-                                     const vx_trace_params* params, vx_batch** out) {
-    VX_REQUIRE(ctx && out && palette, "ctx/out/palette is NULL");
+// Plans the batch for items (tile queue, descriptor table, scratch, launch
+// arguments) and captures its launch, replacing a previous graph. Arguments
+// are checked before anything is changed; a failure after that leaves the
+// batch invalid until the next successful plan.
+static vx_status batch_plan(vx_batch* b, const vx_batch_item* items, uint32_t n) {
+    vx_ctx* ctx = b->ctx;
+    const vx_trace_params* params = &b->params;
     VX_REQUIRE(n >= 1 && n <= 65536 && items, "batch needs 1..65536 items");
-    if (vx_status st = check_params(params)) return st;
-    VX_REQUIRE((params->n_shadow == 0 || params->shadow_march == VX_MARCH_DDA) &&
-                   (params->n_ao == 0 || params->ao_march == VX_MARCH_DDA) &&
-                   (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA),
-               "the batched trace runs the DDA marcher for every ray class");
-    *out = nullptr;
     std::vector<MarchItem> h(n);
     uint64_t tiles = 0;
     bool fetches = false;
@@ -1795,7 +1854,7 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
         T.params = *params;
         T.tiles = tile_map(it.view.width, it.view.height);
         T.pos = it.gbuf.pos; T.nrm = it.gbuf.normal; T.mat = it.gbuf.material;
-        T.palette = palette;
+        T.palette = b->palette;
         T.shadow = it.out.shadow; T.ao = it.out.ao; T.spec = it.out.spec; T.fetches = it.out.fetches;
         fetches = fetches || it.out.fetches;
         h[i] = make_march_item(it.volume, T, (uint32_t)tiles, tile_geom(ctx->tuning, it.view.width, it.view.height));
@@ -1806,27 +1865,41 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
     const size_t slots = (size_t)tiles * (size_t)nrays * 64;
     VX_REQUIRE(slots < (1ull << 32), "batch needs %zu ray slots, more than 2^32", slots);
     if (vx_status st = use_device(ctx)) return st;
-    vx_batch* b = new (std::nothrow) vx_batch();
-    if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
-    std::memset(b, 0, sizeof(*b));
-    b->ctx = ctx;
+    // from here on the batch changes: a previous graph's launches finish first
+    VX_HIP(hipDeviceSynchronize());
+    b->valid = false;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
     b->n = n;
     std::vector<uint16_t> tile_item((size_t)tiles);
     for (uint32_t i = 0; i < n; ++i)
         std::fill(tile_item.begin() + h[i].tile0, tile_item.begin() + h[i].tile0 + h[i].ntiles, (uint16_t)i);
-    hipError_t e = hipMalloc(&b->items, n * sizeof(MarchItem));
+    hipError_t e = hipSuccess;
+    if (b->items_cap < n) {
+        (void)hipFree(b->items);
+        b->items = nullptr;
+        b->items_cap = 0;
+        e = dev_malloc(&b->items, n * sizeof(MarchItem));
+        if (e == hipSuccess) b->items_cap = n;
+    }
     if (e == hipSuccess) e = hipMemcpy(b->items, h.data(), n * sizeof(MarchItem), hipMemcpyHostToDevice);
-    if (e == hipSuccess && tiles) e = hipMalloc(&b->tile_item, tile_item.size() * sizeof(uint16_t));
+    if (e == hipSuccess && b->tile_cap < tiles) {
+        (void)hipFree(b->tile_item);
+        b->tile_item = nullptr;
+        b->tile_cap = 0;
+        e = dev_malloc(&b->tile_item, tile_item.size() * sizeof(uint16_t));
+        if (e == hipSuccess) b->tile_cap = tiles;
+    }
     if (e == hipSuccess && tiles)
         e = hipMemcpy(b->tile_item, tile_item.data(), tile_item.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        batch_free(b);
-        return fail(VX_ERR_OUT_OF_MEMORY, "batch allocation failed: %s", hipGetErrorString(e));
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? VX_ERR_OUT_OF_MEMORY : VX_ERR_DEVICE, "batch allocation failed: %s",
+                    hipGetErrorString(e));
     }
-    if (vx_status st = ensure_march(b->march, slots, fetches)) {
-        batch_free(b);
-        return st;
-    }
+    if (vx_status st = ensure_march(b->march, slots, fetches)) return st;
     MarchArgs& A = b->args;
     std::memset(&A, 0, sizeof(A));
     A.item0 = h[0];
@@ -1841,23 +1914,44 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
     A.deferred_cap = (uint32_t)slots;
     A.res = b->march.res;
     A.fres = fetches ? b->march.fres : nullptr;
-    if (vx_status st = march_shape(ctx, false, b->shape)) {
-        batch_free(b);
-        return st;
-    }
-    // Capture the launch once; vx_batch_launch replays it.
-    e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+    if (vx_status st = march_shape(ctx, false, b->shape)) return st;
+    // capture the launch; vx_batch_launch replays it
+    if (!b->cap) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         hipError_t le = batch_enqueue(b, b->cap);
+        if (le == hipSuccess) le = launch_status();
         e = hipStreamEndCapture(b->cap, &b->graph);
         if (e == hipSuccess) e = le;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        batch_free(b);
         return fail(VX_ERR_DEVICE, "batch graph capture failed: %s", hipGetErrorString(e));
+    }
+    b->captures++;
+    b->valid = true;
+    return VX_OK;
+}
+
+static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
+                                     const vx_trace_params* params, vx_batch** out) {
+    VX_REQUIRE(ctx && out && palette, "ctx/out/palette is NULL");
+    if (vx_status st = check_params(params)) return st;
+    VX_REQUIRE((params->n_shadow == 0 || params->shadow_march == VX_MARCH_DDA) &&
+                   (params->n_ao == 0 || params->ao_march == VX_MARCH_DDA) &&
+                   (params->n_spec == 0 || params->spec_march == VX_MARCH_DDA),
+               "the batched trace runs the DDA marcher for every ray class");
+    *out = nullptr;
+    vx_batch* b = new (std::nothrow) vx_batch();
+    if (!b) return fail(VX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    std::memset(b, 0, sizeof(*b));
+    b->ctx = ctx;
+    b->palette = palette;
+    b->params = *params;
+    if (vx_status st = batch_plan(b, items, n)) {
+        batch_free(b);
+        return st;
     }
     *out = b;
     return VX_OK;
@@ -1865,6 +1959,7 @@ static vx_status vx_batch_create_impl(vx_ctx* ctx, const vx_batch_item* items, u
 
 extern "C" vx_status vx_batch_launch(vx_batch* b, void* stream) {
     VX_REQUIRE(b, "batch is NULL");
+    if (!b->valid) return fail(VX_ERR_STATE, "the batch's last update failed; update it again or destroy it");
     if (vx_status st = use_device(b->ctx)) return st;
     VX_HIP(hipGraphLaunch(b->exec, S(stream)));
     return VX_OK;
@@ -1872,6 +1967,7 @@ extern "C" vx_status vx_batch_launch(vx_batch* b, void* stream) {
 
 static vx_status vx_batch_launch_direct_impl(vx_batch* b, void* stream) {
     VX_REQUIRE(b, "batch is NULL");
+    if (!b->valid) return fail(VX_ERR_STATE, "the batch's last update failed; update it again or destroy it");
     if (vx_status st = use_device(b->ctx)) return st;
     vx_ctx* ctx = b->ctx;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1927,6 +2023,18 @@ extern "C" vx_status vx_shade_backward_batch(vx_ctx* ctx, const vx_shade_item* i
 extern "C" vx_status vx_batch_create(vx_ctx* ctx, const vx_batch_item* items, uint32_t n, const float* palette,
                                      const vx_trace_params* params, vx_batch** out) {
     return guarded("vx_batch_create", [&] { return vx_batch_create_impl(ctx, items, n, palette, params, out); });
+}
+
+extern "C" vx_status vx_batch_update(vx_batch* b, const vx_batch_item* items, uint32_t n) {
+    VX_REQUIRE(b, "batch is NULL");
+    return guarded("vx_batch_update", [&] { return batch_plan(b, items, n); });
+}
+
+extern "C" vx_status vx_batch_info(const vx_batch* b, uint32_t* nitems, uint32_t* captures) {
+    VX_REQUIRE(b, "batch is NULL");
+    if (nitems) *nitems = b->n;
+    if (captures) *captures = b->captures;
+    return VX_OK;
 }
 
 extern "C" vx_status vx_batch_launch_direct(vx_batch* b, void* stream) {
