@@ -906,8 +906,8 @@ struct MarchRay {
     uint32_t meta;
     float maxt;
     uint32_t slot;
-    uint32_t ci;           // grid word held in cw
-    uint64_t cw;
+    uint32_t ci;           // index of the grid word at the lane's position
+    uint64_t cw;           // grid word of the lane's voxel
 };
 
 // Lane coordinates: x = X + 4, y = (Y + 4) << 2, z = (Z + 4) << 4 for fine
@@ -932,6 +932,51 @@ VXD uint32_t grid_word(int32_t x, int32_t y, int32_t z, uint32_t gx, uint32_t gx
     asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"((uint32_t)(y >> 4)), "s"(gx), "v"((uint32_t)(x >> 2)));
     asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"((uint32_t)(z >> 6)), "s"(gxy), "v"(a));
     return r;
+}
+
+// Position and t_max steps along x / y / z of the lanes in e0 / e1 / e2 that
+// are also in m. The adds run with those lanes as exec, which the block
+// restores to `full` (the wave's exec where it is called) before it ends:
+// each costs one add, instead of an add and a select.
+VXD void move_axes(MarchRay& R, uint64_t e0, uint64_t e1, uint64_t e2, uint64_t m, uint64_t full) {
+    asm volatile(
+        "s_and_b64 exec, %[e0], %[m]\n\t"
+        "v_add_u32 %[x], %[x], %[sx]\n\t"
+        "s_and_b64 exec, %[e1], %[m]\n\t"
+        "v_add_u32 %[y], %[y], %[sy]\n\t"
+        "s_and_b64 exec, %[e2], %[m]\n\t"
+        "v_add_u32 %[z], %[z], %[sz]\n\t"
+        "s_mov_b64 exec, %[full]"
+        : [x] "+v"(R.x), [y] "+v"(R.y), [z] "+v"(R.z)
+        : [e0] "s"(e0), [e1] "s"(e1), [e2] "s"(e2), [m] "s"(m), [full] "s"(full), [sx] "v"(R.sx),
+          [sy] "v"(R.sy), [sz] "v"(R.sz)
+        : "scc");
+}
+VXD void advance_tmax(MarchRay& R, uint64_t e0, uint64_t e1, uint64_t e2, uint64_t m, uint64_t full) {
+    asm volatile(
+        "s_and_b64 exec, %[e0], %[m]\n\t"
+        "v_add_f32 %[t0], %[t0], %[d0]\n\t"
+        "s_and_b64 exec, %[e1], %[m]\n\t"
+        "v_add_f32 %[t1], %[t1], %[d1]\n\t"
+        "s_and_b64 exec, %[e2], %[m]\n\t"
+        "v_add_f32 %[t2], %[t2], %[d2]\n\t"
+        "s_mov_b64 exec, %[full]"
+        : [t0] "+v"(R.tm0), [t1] "+v"(R.tm1), [t2] "+v"(R.tm2)
+        : [e0] "s"(e0), [e1] "s"(e1), [e2] "s"(e2), [m] "s"(m), [full] "s"(full), [d0] "v"(R.td0),
+          [d1] "v"(R.td1), [d2] "v"(R.td2)
+        : "scc");
+}
+
+// meta += inc in the lanes of m (exec restored before the block ends).
+VXD void add_masked(uint32_t& meta, uint32_t inc, uint64_t m) {
+    uint64_t sv;
+    asm volatile(
+        "s_mov_b64 %[sv], exec\n\t"
+        "s_mov_b64 exec, %[m]\n\t"
+        "v_add_u32 %[v], %[i], %[v]\n\t"
+        "s_mov_b64 exec, %[sv]"
+        : [v] "+v"(meta), [sv] "=&s"(sv)
+        : [m] "s"(m), [i] "s"(inc));
 }
 
 // Occupancy: 7 waves per SIMD (72 VGPRs, 140 KB of LDS rings per CU). A wave
@@ -963,10 +1008,14 @@ k_march(MarchArgs A) {
     P.org = mk3(0.0f, 0.0f, 0.0f); P.pk = 0; P.i = 0;
     bool pvalid = false;
     // this lane's ray
+    // A lane without a ray keeps x = y = z = 0 and ci = 0 = the grid word of
+    // that position: the march's steps leave it in place and load nothing.
     MarchRay R;
-    R.tm0 = R.tm1 = R.tm2 = R.td0 = R.td1 = R.td2 = 0.0f;
-    R.x = R.y = R.z = 4; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.ci = 0; R.cw = 0;
-    bool hasray = false, running = false, stopped = false;
+    R.tm0 = 0.0f; R.tm1 = 1.0f; R.tm2 = 2.0f;
+    R.td0 = R.td1 = R.td2 = 0.0f;
+    R.x = R.y = R.z = 0; R.sx = R.sy = R.sz = 1; R.meta = 0; R.maxt = 0.0f; R.slot = 0; R.ci = 0; R.cw = 0;
+    bool hasray = false, running = false;
+    const uint64_t lane_bit = 1ull << lane;
     // the wave's item (its rays in flight all belong to it)
     uint32_t citem = SINGLE ? 0u : ~0u, nitem = 0;
     bool pending = false;
@@ -981,6 +1030,21 @@ k_march(MarchArgs A) {
             // above it: the SIMD then favours waves that are marching)
             __builtin_amdgcn_s_setprio(0);
             if (hasray && !running) {
+                // The march moved the ray's position ahead in its last step
+                // (its t_max did not advance): step it back to the voxel that
+                // step tested. A ray whose step ran past maxt or the step limit
+                // tested a free voxel, so the ray sits in an occupied voxel
+                // exactly when it stopped there.
+                {
+                    const float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
+                    R.x = R.tm0 == tmin ? R.x - R.sx : R.x;
+                    R.y = R.tm1 == tmin ? R.y - R.sy : R.y;
+                    R.z = R.tm2 == tmin ? R.z - R.sz : R.z;
+                }
+                R.ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
+                R.cw = grid[R.ci];
+                const bool stopped =
+                    (int64_t)(R.cw << (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) < 0;
                 const uint32_t kind = (R.meta >> 16) & 3u, n = (R.meta >> 20) - 0x700u - (stopped ? 1u : 0u);
                 const MarchItem& I = march_item_uniform<SINGLE>(A, citem);
                 int32_t term = 0;
@@ -1010,7 +1074,9 @@ k_march(MarchArgs A) {
                                     (int32_t)ray_fetches(term, (int32_t)n), vox, tt, nrm);
                 }
                 hasray = false;
-                stopped = false;
+                R.x = R.y = R.z = 0;
+                R.ci = 0;
+                R.tm0 = 0.0f; R.tm1 = 1.0f; R.tm2 = 2.0f;
             }
             // ---- generate until the ring holds 64 rays or the work is out
             while (tail - head < 64u && !gen_done) {
@@ -1118,54 +1184,65 @@ k_march(MarchArgs A) {
                     R.cw = grid[R.ci];
                     hasray = true;
                     running = true;
-                    stopped = false;
                 }
             }
             head += min((uint32_t)__popcll(needm), avail);
             if (__ballot(hasray) == 0 && gen_done && head == tail) break;
         }
-        // ---- march
+        // ---- march: STEPS steps of the running lanes. The wave keeps which
+        // lanes run as a lane mask; every lane evaluates the step and only the
+        // masked lanes change. A running lane moves its position first, to
+        // the voxel the step enters when its own voxel is free, and fetches
+        // that voxel's grid word right away: the fetch is in flight while the
+        // step tests the current voxel (with the word fetched one step
+        // earlier), and the next step uses it. t_max advances only when the
+        // voxel was free and the ray goes on; a ray that ends keeps its
+        // t_max, and the finish steps its position back. A lane that does not
+        // run keeps its position (a lane without a ray sits at word 0).
         __builtin_amdgcn_s_setprio(1);
+        uint64_t run = __ballot(running);
+        const uint64_t full = __builtin_amdgcn_read_exec();
+        uint64_t w = R.cw;
 #pragma unroll
         for (int u = 0; u < STEPS; ++u) {
-            if (running) {
-                // grid words are bit-reversed: voxel bit b sits at bit 63 - b. An
-                // occupied cell stops the ray where it is: no axis steps, and its
-                // step count (counted here) loses that step when it finishes.
-                const bool occ =
-                    (int64_t)(R.cw << (cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u)) < 0;
-                const float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
-                const bool s0 = R.tm0 == tmin && !occ, s1 = R.tm1 == tmin && !occ, s2 = R.tm2 == tmin && !occ;
-                float bt = tmin;
-                // two or three axes step together (rare): the median equals the minimum
-                if (__builtin_expect(__builtin_amdgcn_fmed3f(R.tm0, R.tm1, R.tm2) == tmin, 0)) {
+            const uint32_t b = cell_shift((uint32_t)R.x, (uint32_t)R.y, (uint32_t)R.z) & 63u;
+            const float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
+            const uint64_t m0 = __ballot(R.tm0 == tmin), m1 = __ballot(R.tm1 == tmin), m2 = __ballot(R.tm2 == tmin);
+            move_axes(R, m0, m1, m2, run, full);
+            // the word is fetched only when the position enters another cell
+            const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
+            uint64_t nw = w;
+            if (ci != R.ci) nw = grid[ci];
+            R.ci = ci;
+            // two or three axes step together (rare): best_t sums their t. (A
+            // lane without a ray holds distinct t_max values, so it never
+            // counts here.)
+            float bt = tmin;
+            if (__builtin_expect(((m0 & m1) | (m2 & (m0 | m1))) != 0, 0)) {
+                const uint64_t tie = (m0 & m1) | (m2 & (m0 | m1));
+                if (tie & lane_bit) {
                     const float t2 = tmin + tmin;
-                    bt = (R.tm0 == tmin && R.tm1 == tmin && R.tm2 == tmin) ? t2 + tmin : t2;
+                    bt = (m0 & m1 & m2 & lane_bit) ? t2 + tmin : t2;
                 }
-                R.x = s0 ? R.x + R.sx : R.x;
-                R.y = s1 ? R.y + R.sy : R.y;
-                R.z = s2 ? R.z + R.sz : R.z;
-                R.tm0 = s0 ? R.tm0 + R.td0 : R.tm0;
-                R.tm1 = s1 ? R.tm1 + R.td1 : R.tm1;
-                R.tm2 = s2 ? R.tm2 + R.td2 : R.tm2;
-                // A ray enters the march between blocks of STEPS steps with a
-                // count of 0 and steps once per iteration while it runs, so its
-                // count reaches 256 (a multiple of STEPS) only at a block's last
-                // step: the limit is tested there alone.
-                if (COUNT) R.meta += 1u << 20;
-                else if (u == STEPS - 1) R.meta += (uint32_t)STEPS << 20;
-                running = !occ && (u != STEPS - 1 || (int32_t)R.meta >= 0) && bt < R.maxt;
-                stopped = occ;
-                // the word is fetched only when the ray enters another cell
-                uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
-                if (ci != R.ci) R.cw = grid[ci];
-                // R.ci takes ci's register on both paths: without the empty asm
-                // the compiler reuses R.ci's where they are equal and copies ci
-                // into it inside the branch, one move per step
-                asm("" : "+v"(ci));
-                R.ci = ci;
             }
+            // A ray enters the march between blocks of STEPS steps with a
+            // count of 0 and steps once per iteration while it runs, so its
+            // count reaches 256 (a multiple of STEPS) only at a block's last
+            // step: the limit is tested there alone.
+            if (COUNT) add_masked(R.meta, 1u << 20, run);
+            else if (u == STEPS - 1) add_masked(R.meta, (uint32_t)STEPS << 20, run);
+            uint64_t ok = __ballot(bt < R.maxt);
+            if (u == STEPS - 1) ok &= __ballot((int32_t)R.meta >= 0);
+            // grid words are bit-reversed: voxel bit b sits at bit 63 - b. An
+            // occupied voxel stops the ray there; its step count (counted
+            // here) loses that step when it finishes.
+            const bool occ = (int64_t)(w << b) < 0;
+            run = run & ~__ballot(occ) & ok;
+            advance_tmax(R, m0, m1, m2, run, full);
+            w = nw;
         }
+        R.cw = w;
+        running = (run & lane_bit) != 0;
     }
 }
 
