@@ -90,20 +90,32 @@ def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
 
 
 def limiter(counters, kernel_ms):
-    """What the rocprofv3 counters of this kernel source show bounds k_march:
-    measured HBM traffic against the HBM roofline, and vector-instruction issue
-    (SQ_INSTS_VALU wave instructions x 4 cycles per wave64 instruction over
-    1024 SIMDs at 2.4 GHz) against the kernel time; None without counters."""
+    """What the rocprofv3 counters of this kernel source show bounds k_march,
+    each as a fraction of the kernel time (1.0 = that resource busy for the
+    whole launch): vector and scalar instruction issue (SQ_INSTS_VALU /
+    SQ_INSTS_SALU wave instructions, one per SIMD every 4 cycles, over 1024
+    SIMDs at the measured engine clock), the vector-memory address and data
+    paths (TA / TD busy cycles per CU), waves waiting on memory counters
+    (SQ_WAIT_ANY / SQ_WAVE_CYCLES) and measured HBM traffic against the HBM
+    roofline. `bound` names the largest. None without counters."""
     if not counters:
         return None
     t = kernel_ms * 1e-3
-    hbm = counters["traffic_bytes_per_launch"] / t / (HBM_PEAK_GBS * 1e9)
-    issue_s = counters["sq_insts_valu_per_launch"] * 4.0 / (MI355X_SIMDS * MI355X_CLOCK_HZ)
-    return {"bound": "valu_issue" if issue_s / t > hbm else "hbm",
-            "valu_issue_ms": round(issue_s * 1e3, 4), "valu_issue_frac": round(issue_s / t, 4),
-            "hbm_traffic_frac": round(hbm, 4), "l2_hit_rate": counters["l2_hit_rate"],
-            "lane_use": counters.get("lane_use"),
-            "counters": os.path.relpath(COUNTERS, ROOT) + " @ kernel source " + counters["kernel_src_sha16"]}
+    clock = counters.get("engine_clock_hz") or MI355X_CLOCK_HZ
+    cyc = t * clock
+    fr = {"valu_issue": counters["sq_insts_valu_per_launch"] * 4.0 / MI355X_SIMDS / cyc,
+          "salu_issue": counters["sq_insts_salu_per_launch"] * 4.0 / MI355X_SIMDS / cyc,
+          "hbm": counters["traffic_bytes_per_launch"] / t / (HBM_PEAK_GBS * 1e9)}
+    if "ta_busy_per_cu" in counters:
+        fr["ta_busy"] = counters["ta_busy_per_cu"] / cyc
+        fr["td_busy"] = counters["td_busy_per_cu"] / cyc
+    out = {"bound": max(fr, key=fr.get)}
+    out.update({k + "_frac": round(v, 4) for k, v in fr.items()})
+    out.update({"engine_clock_ghz": round(clock / 1e9, 3), "l2_hit_rate": counters["l2_hit_rate"],
+                "l1_hit_rate": counters.get("l1_hit_rate"), "mem_wait_frac": counters.get("wait_frac_mem"),
+                "issue_wait_frac": counters.get("wait_frac"),
+                "counters": os.path.relpath(COUNTERS, ROOT) + " @ kernel source " + counters["kernel_src_sha16"]})
+    return out
 
 
 def cpu_model():
@@ -431,11 +443,14 @@ def summarize_profiles(d):
     `bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-extras --copy-probe`
     under d: pmc_fetch (FETCH_SIZE), pmc_write (WRITE_SIZE), pmc_tcc
     (TCC_HIT_sum TCC_MISS_sum), pmc_sq (SQ_INSTS_VALU SQ_INSTS_SALU
-    SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_WAIT_INST_ANY SQ_WAVE_CYCLES).
-    The timed launches (k_march without fetch counts, the first of them
-    excluded: cold caches) are averaged. FETCH_SIZE / WRITE_SIZE are scaled by
-    what they read for the 1 GiB device copy of --copy-probe. Writes the
-    counters file and returns it."""
+    SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_SCA SQ_WAIT_INST_ANY SQ_WAIT_ANY
+    SQ_WAVE_CYCLES SQ_BUSY_CYCLES) and pmc_ta (TA_TA_BUSY_sum TD_TD_BUSY_sum
+    TCP_TOTAL_CACHE_ACCESSES_sum TCP_TCC_READ_REQ_sum GRBM_GUI_ACTIVE). The
+    timed launches (k_march without fetch counts, the first of them excluded:
+    cold caches) are averaged. FETCH_SIZE / WRITE_SIZE are scaled by what they
+    read for the 1 GiB device copy of --copy-probe; the engine clock is
+    GRBM_GUI_ACTIVE (summed over the 8 XCDs) over the launch's own duration
+    in the same pass. Writes the counters file and returns it."""
     march = "k_march<256, 8, true, false>"
 
     def per_launch(sub, names):
@@ -448,27 +463,49 @@ def summarize_profiles(d):
     f, fc, n1 = per_launch("pmc_fetch", ["FETCH_SIZE"])
     w, wc, n2 = per_launch("pmc_write", ["WRITE_SIZE"])
     t, _, n3 = per_launch("pmc_tcc", ["TCC_HIT_sum", "TCC_MISS_sum"])
-    q, _, n4 = per_launch("pmc_sq", ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU",
-                                     "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"])
+    q, _, n4 = per_launch("pmc_sq", ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_SCA",
+                                     "SQ_WAIT_INST_ANY", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES"])
+    a, _, n5 = per_launch("pmc_ta", ["TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "TCP_TOTAL_CACHE_ACCESSES_sum",
+                                     "TCP_TCC_READ_REQ_sum", "GRBM_GUI_ACTIVE"])
+    dur = _dispatch_ns(os.path.join(d, "pmc_ta", "run_counter_collection.csv"), march)
     gib = float(1 << 30)
     fscale = gib / (max(c["FETCH_SIZE"] for c in fc) * 1024.0)
     wscale = gib / (max(c["WRITE_SIZE"] for c in wc) * 1024.0)
     fetch_b = f["FETCH_SIZE"] * 1024.0 * fscale
     write_b = w["WRITE_SIZE"] * 1024.0 * wscale
+    clock = a["GRBM_GUI_ACTIVE"] / 8.0 / (dur * 1e-9) if dur else None
     c = {"kernel": march, "workload": "F: 512^3, 1920x1080, 1 shadow + 16 AO(32) + 1 spec DDA rays per hit pixel",
-         "kernel_src_sha16": kernel_src_sha16(), "launches_averaged": [n1, n2, n3, n4],
+         "kernel_src_sha16": kernel_src_sha16(), "launches_averaged": [n1, n2, n3, n4, n5],
          "fetch_size_scale": round(fscale, 4), "write_size_scale": round(wscale, 4),
          "hbm_fetch_bytes_per_launch": int(fetch_b), "hbm_write_bytes_per_launch": int(write_b),
          "traffic_bytes_per_launch": int(fetch_b + write_b),
          "tcc_hit_per_launch": int(t["TCC_HIT_sum"]), "tcc_miss_per_launch": int(t["TCC_MISS_sum"]),
          "l2_hit_rate": round(t["TCC_HIT_sum"] / (t["TCC_HIT_sum"] + t["TCC_MISS_sum"]), 4),
+         "l1_accesses_per_launch": int(a["TCP_TOTAL_CACHE_ACCESSES_sum"]),
+         "l1_hit_rate": round(1.0 - a["TCP_TCC_READ_REQ_sum"] / a["TCP_TOTAL_CACHE_ACCESSES_sum"], 4),
          "sq_insts_valu_per_launch": int(q["SQ_INSTS_VALU"]), "sq_insts_salu_per_launch": int(q["SQ_INSTS_SALU"]),
-         "lane_use": round(q["SQ_THREAD_CYCLES_VALU"] / (64.0 * q["SQ_ACTIVE_INST_VALU"]), 4),
-         "wait_frac": round(q["SQ_WAIT_INST_ANY"] / q["SQ_WAVE_CYCLES"], 4)}
+         "wait_frac": round(q["SQ_WAIT_INST_ANY"] / q["SQ_WAVE_CYCLES"], 4),
+         "wait_frac_mem": round(q["SQ_WAIT_ANY"] / q["SQ_WAVE_CYCLES"], 4),
+         "ta_busy_per_cu": round(a["TA_TA_BUSY_sum"] / 256.0), "td_busy_per_cu": round(a["TD_TD_BUSY_sum"] / 256.0),
+         "engine_clock_hz": round(clock) if clock else None,
+         "kernel_ns_in_ta_pass": round(dur) if dur else None}
     with open(COUNTERS, "w") as fo:
         json.dump(c, fo, indent=1)
         fo.write("\n")
     return c
+
+
+def _dispatch_ns(path, kernel):
+    """Average duration (ns) of the kernel's dispatches in a counter pass,
+    the first excluded."""
+    import csv
+    seen = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if kernel in r["Kernel_Name"]:
+                seen[int(r["Dispatch_Id"])] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    v = [seen[k] for k in sorted(seen)][1:]
+    return sum(v) / len(v) if v else None
 
 
 def main():
