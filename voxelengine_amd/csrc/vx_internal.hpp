@@ -132,15 +132,16 @@ struct DepthArgs {
 struct LightRec {
     uint32_t pixel;
     uint32_t light;
+    uint32_t slot;             // the ray's term slot: its pixel's first record + the light's rank
 };
 
 struct LightListArgs {
     uint32_t* total;           // rays listed (one atomic per tile)
     uint32_t* first;           // per pixel: its first record
     uint32_t* count;           // per pixel: its in-range lights
-    LightRec* rec;             // the records, a pixel's in draw order
-    float* term;               // 3 per record: the light's shadowed direct term
-    uint16_t* rfetch;          // per record: the marcher's fetch count, or null
+    LightRec* rec;             // the records, per tile light by light (pixels in order)
+    float* term;               // 3 per slot: the light's shadowed direct term
+    uint16_t* rfetch;          // per slot: the marcher's fetch count, or null
 };
 
 // A model's three mips (X-fastest bytes) and mip-0 dims.

@@ -1632,10 +1632,14 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
 // The light pass as a ray list: a pixel's (pixel, light) shadow rays are
 // listed, marched one per lane, and their terms folded per pixel in light
 // order. Pass 1 counts each pixel's in-range lights and gives each 16x16
-// tile a contiguous range of the list (one atomic per tile); pass 2 writes
-// the records; pass 3 marches every record and stores its term; pass 4 adds
-// a pixel's terms to its light buffer value in list order, which is draw
-// order. The arithmetic is k_lights' (the same device functions).
+// tile a contiguous range of the list (one atomic per tile) and each pixel
+// a contiguous range of term slots in it; pass 2 writes the tile's records
+// light by light, each light's in-range pixels in pixel order, so that the
+// 64 rays a wave marches run from neighbouring pixels to one light (similar
+// directions and lengths); pass 3 marches every record and stores its term in
+// its slot; pass 4 adds a pixel's terms to its light buffer value in slot
+// order, which is draw order. The arithmetic is k_lights' (the same device
+// functions).
 // ---------------------------------------------------------------------------
 template <bool WRITE>
 VXD void light_list_tile(const LightArgs& A, const LightListArgs& B) {
@@ -1684,16 +1688,32 @@ VXD void light_list_tile(const LightArgs& A, const LightListArgs& B) {
             B.count[i] = c;
         }
     } else {
-        if (!valid) return;
-        uint32_t j = B.first[i];
+        // thread 0's pixel is the tile's first (always in view): its first
+        // slot is the tile's first record
+        const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        const uint64_t lane_lt = (1ull << lane) - 1ull;
+        if (threadIdx.x == 0) tile_base_s = B.first[i];
+        uint32_t slot = valid ? B.first[i] : 0u;
+        __syncthreads();
+        uint32_t off = tile_base_s;
         for (uint32_t k = 0; k < n; ++k) {
             const uint32_t l = list_s[k];
             f3 ld;
             float dist;
-            if (!light_in_range(A.lights[__builtin_amdgcn_readfirstlane(l)], wpos, ld, dist)) continue;
-            B.rec[j].pixel = (uint32_t)i;
-            B.rec[j].light = l;
-            ++j;
+            const bool in = valid && light_in_range(A.lights[__builtin_amdgcn_readfirstlane(l)], wpos, ld, dist);
+            const uint64_t m = __ballot(in);
+            if (lane == 0) wsum_s[wave] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t at = off + (uint32_t)__popcll(m & lane_lt);
+            for (uint32_t w = 0; w < wave; ++w) at += wsum_s[w];
+            if (in) {
+                B.rec[at].pixel = (uint32_t)i;
+                B.rec[at].light = l;
+                B.rec[at].slot = slot;
+                ++slot;
+            }
+            off += (wsum_s[0] + wsum_s[1]) + (wsum_s[2] + wsum_s[3]);
+            __syncthreads();
         }
     }
 }
@@ -1721,10 +1741,10 @@ __global__ void __launch_bounds__(256) k_light_march(OccView V, LightArgs A, Lig
             march_dist(V, L.type == VX_LIGHT_SPOT ? A.params.spot_march : A.params.point_march, o, wdn, hit_dist, nf, hit);
         float term[3];
         light_term(L, P, ld, dist, (d < hit_dist) ? 0.0f : 1.0f, term);
-        B.term[3 * (size_t)j + 0] = term[0];
-        B.term[3 * (size_t)j + 1] = term[1];
-        B.term[3 * (size_t)j + 2] = term[2];
-        if (B.rfetch) B.rfetch[j] = (uint16_t)nf;
+        B.term[3 * (size_t)r.slot + 0] = term[0];
+        B.term[3 * (size_t)r.slot + 1] = term[1];
+        B.term[3 * (size_t)r.slot + 2] = term[2];
+        if (B.rfetch) B.rfetch[r.slot] = (uint16_t)nf;
     }
 }
 
