@@ -379,6 +379,46 @@ def bench_lights(ctx, occ, vd, g, pal_d, dims, reps=10):
             "pixel_loop_grays_per_s": round(rays / (times["pixel_loop"] * 1e-3) / 1e9, 3)}
 
 
+def bench_reflection_ssao(ctx, occ, vd, g, pal_d, reps=10):
+    """The reflection pass (LightReflection.frag: the reference's Sparse
+    marcher to 256, reprojection into a synthetic light buffer, procedural
+    sky) and the screen-space occlusion (calculateOcclusion) over the headline
+    frame, from the G-buffer's D24 depth attachment. One reflection ray is
+    marched per hit pixel whose depth is below 0.999."""
+    import torch
+    from voxelengine_amd import api
+    npx = vd["width"] * vd["height"]
+    depth = torch.zeros(npx, dtype=torch.float32, device="cuda")
+    api.gbuf_depth(ctx, vd, g, depth, volume_rid=1)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    light = torch.rand((npx, 4), dtype=torch.float32, device="cuda", generator=gen)
+    out = torch.zeros((npx, 4), dtype=torch.float32, device="cuda")
+    t = torch.zeros(npx, dtype=torch.float32, device="cuda")
+    ss = torch.zeros(npx, dtype=torch.float32, device="cuda")
+    api.reflection(ctx, occ, vd, g, pal_d, light, depth, out, t=t, seed=3, frame=1)
+    torch.cuda.synchronize()
+    rays = int((~torch.isnan(t)).sum().item())
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        api.reflection(ctx, occ, vd, g, pal_d, light, depth, out, seed=3, frame=1)
+    e1.record()
+    torch.cuda.synchronize()
+    refl_ms = e0.elapsed_time(e1) / reps
+    api.ssao(ctx, vd, g, depth, ss, seed=3)
+    e0.record()
+    for _ in range(reps):
+        api.ssao(ctx, vd, g, depth, ss, seed=3)
+    e1.record()
+    torch.cuda.synchronize()
+    ssao_ms = e0.elapsed_time(e1) / reps
+    return {"reflection": {"march": "sparse", "maxt": 256.0, "sky": "procedural", "rays": rays,
+                           "ms": round(refl_ms, 4), "grays_per_s": round(rays / (refl_ms * 1e-3) / 1e9, 3)},
+            "ssao": {"samples_per_pixel": 4, "ms": round(ssao_ms, 4),
+                     "mpx_per_s": round(npx / (ssao_ms * 1e-3) / 1e6, 1),
+                     "occluded_fraction": round(float((ss < 1).float().mean().item()), 4)}}
+
+
 def bench_prefabs(ctx, reps=5):
     """The reference's FarmHouse and ModernHouse prefabs (files rebuilt from
     tests/golden/prefabs.npz) through the loaders into the 524x188x524 volume:
@@ -679,6 +719,7 @@ def main():
     if not args.no_extras:
         log("light pass ...")
         out["lights"] = bench_lights(ctx, occ, vd, g, pal_d, cfg["dims"])
+        out.update(bench_reflection_ssao(ctx, occ, vd, g, pal_d))
         out["batched"] = bench_batched(ctx, pal_d)
         log("prefabs ...")
         out["prefabs"] = bench_prefabs(ctx)
