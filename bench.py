@@ -91,8 +91,8 @@ def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
 
 def limiter(counters, kernel_ms):
     """What the rocprofv3 counters of this kernel source show bounds k_march,
-    each as a fraction of the kernel time (1.0 = that resource busy for the
-    whole launch): vector and scalar instruction issue (SQ_INSTS_VALU /
+    each as a fraction of the counted launch's time (1.0 = that resource busy
+    for the whole launch): vector and scalar instruction issue (SQ_INSTS_VALU /
     SQ_INSTS_SALU wave instructions, one per SIMD every 4 cycles, over 1024
     SIMDs at the measured engine clock), the vector-memory address and data
     paths (TA / TD busy cycles per CU), waves waiting on memory counters
@@ -100,7 +100,9 @@ def limiter(counters, kernel_ms):
     roofline. `bound` names the largest. None without counters."""
     if not counters:
         return None
-    t = kernel_ms * 1e-3
+    # the fractions of the counted dispatch itself (its duration and clock),
+    # so that they stay consistent when this run's box clocks differently
+    t = (counters.get("kernel_ns_in_ta_pass") or kernel_ms * 1e6) * 1e-9
     clock = counters.get("engine_clock_hz") or MI355X_CLOCK_HZ
     cyc = t * clock
     fr = {"valu_issue": counters["sq_insts_valu_per_launch"] * 4.0 / MI355X_SIMDS / cyc,
