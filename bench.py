@@ -225,7 +225,7 @@ def bench_large(ctx, pal_d, reps=3):
     batch, tuned, tune_ms = autotune.tune_batch(
         ctx, lambda: api.Batch(ctx, items, pal_d, params), "bench-L", reps=1, force=True,
         cache_path=os.path.join(ROOT, "gpurun_out", "tune_cache.json"),
-        candidates=[c for c in autotune.MARCH_CANDIDATES if c["refill"] == 16 and c["steps"] == 8])
+        candidates=autotune.BATCH_CANDIDATES)
     batch.launch()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

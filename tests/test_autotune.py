@@ -15,10 +15,12 @@ class FakeCtx:
         self.tuning = dict(kw)
 
 
-def test_candidates_cover_tile_shapes_orders_and_refill():
+def test_candidates_cover_tile_shapes_orders_and_chunks():
     c = autotune.MARCH_CANDIDATES
     assert {x["tile"] for x in c} == {0, 1, 2} and {x["order"] for x in c} == {0, 1}
-    assert len({(x["refill"], x["steps"]) for x in c}) == 3 and len(c) == 18
+    assert {x["chunks"] for x in c} == {0, 1, 3} and len(c) == 18
+    # a batch takes one queue item per tile: its search skips the chunk knob
+    assert autotune.BATCH_CANDIDATES == [x for x in c if x["chunks"] == 0] and len(autotune.BATCH_CANDIDATES) == 6
     assert autotune.CANDIDATES[0] == dict(kernel=1)
 
 

@@ -89,7 +89,7 @@ def test_batch_rejects_reference_marchers(ctx):
 
 
 def test_every_autotuner_candidate_gives_the_same_bits(ctx, tmp_path):
-    """Each tile shape, queue order and refill setting the autotuner tries,
+    """Each tile shape, queue order and queue-items-per-tile setting the autotuner tries,
     for a batch and for the plain trace, equals the default launch bit for
     bit; tune_batch returns a working batch under the winning tuning."""
     from voxelengine_amd import api, autotune, configs, views
@@ -124,7 +124,7 @@ def test_every_autotuner_candidate_gives_the_same_bits(ctx, tmp_path):
         batch, best, times = autotune.tune_batch(
             ctx, lambda: api.Batch(ctx, [(it["occ"], it["vd"], it["g"], o) for it, o in zip(items, outs)], pal, params),
             "test-batch", cache_path=str(tmp_path / "tune.json"), reps=1)
-        assert len(times) == len(autotune.MARCH_CANDIDATES) and best in autotune.MARCH_CANDIDATES
+        assert len(times) == len(autotune.BATCH_CANDIDATES) and best in autotune.BATCH_CANDIDATES
         batch.launch()
         torch.cuda.synchronize()
         for k, o in enumerate(outs):

@@ -15,7 +15,10 @@ TUNINGS = [dict(kernel=2),
            dict(kernel=2, block=512, refill=1, steps=2),
            dict(kernel=2, refill=64, steps=8, tile=1),
            dict(kernel=2, refill=7, steps=4, blocks_per_cu=1, tile=2, order=1),
-           dict(kernel=2, tile=0, order=1)]
+           dict(kernel=2, tile=0, order=1),
+           dict(kernel=2, chunks=1),
+           dict(kernel=2, chunks=5, tile=1),
+           dict(kernel=2, chunks=64, order=1, refill=3)]
 
 
 @pytest.fixture(scope="module")

@@ -91,7 +91,8 @@ class ShadeItem(C.Structure):
 
 class TraceTuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
-                ("steps", C.c_int32), ("tile", C.c_int32), ("order", C.c_int32), ("lights", C.c_int32)]
+                ("steps", C.c_int32), ("tile", C.c_int32), ("order", C.c_int32), ("lights", C.c_int32),
+                ("chunks", C.c_int32)]
 
 
 class VoxCmd(C.Structure):

@@ -41,8 +41,8 @@ class Context:
         self.h = h
         self.device = device
 
-    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, steps=0, tile=0, order=0, lights=0):
-        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, steps, tile, order, lights)
+    def set_tuning(self, kernel=0, block=0, blocks_per_cu=0, refill=0, steps=0, tile=0, order=0, lights=0, chunks=0):
+        t = L.TraceTuning(kernel, block, blocks_per_cu, refill, steps, tile, order, lights, chunks)
         check(L.lib().vx_ctx_set_trace_tuning(self.h, C.byref(t)))
 
     def reset_tuning(self):
@@ -62,7 +62,7 @@ class Context:
         t = L.TraceTuning()
         check(L.lib().vx_ctx_get_trace_tuning(self.h, C.byref(t)))
         return dict(kernel=t.kernel, block=t.block, blocks_per_cu=t.blocks_per_cu, refill=t.refill, steps=t.steps,
-                    tile=t.tile, order=t.order, lights=t.lights)
+                    tile=t.tile, order=t.order, lights=t.lights, chunks=t.chunks)
 
     def close(self):
         if self.h:

@@ -5,9 +5,9 @@ HIP events, keeps the fastest and caches the choice per workload key in a
 JSON file (VX_TUNE_CACHE, default ~/.cache/voxelengine_amd/tune.json). The
 knobs are the tile shape (8x8, 16x4 or 4x16 pixels per 64-ray batch), the
 tile queue order (rows of tiles, or rows of 8x8-tile blocks so that the tiles
-an XCD takes in turn cover a square screen region), the lane-refill threshold
-and the DDA steps between refill checks; for a plain trace the per-pixel
-kernel is a candidate too. Every candidate computes identical bits; only the
+an XCD takes in turn cover a square screen region) and the queue items per
+tile (the library's own choice from the launch's size, one item per tile, or
+three); for a plain trace the per-pixel kernel is a candidate too. Every candidate computes identical bits; only the
 time differs. A batched launch is captured with the tuning current when it
 is created, so tune_batch creates one batch per candidate and returns the
 fastest one.
@@ -20,10 +20,11 @@ import os
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "voxelengine_amd", "tune.json")
 
-# tile shape x queue order x (refill, steps)
-MARCH_CANDIDATES = [dict(kernel=2, tile=t, order=o, refill=r, steps=st)
-                    for t, o, (r, st) in itertools.product((0, 1, 2), (0, 1), ((16, 8), (32, 8), (16, 4)))]
+# tile shape x queue order x queue items per tile (0: the library's choice)
+MARCH_CANDIDATES = [dict(kernel=2, tile=t, order=o, chunks=c)
+                    for t, o, c in itertools.product((0, 1, 2), (0, 1), (0, 1, 3))]
 CANDIDATES = [dict(kernel=1)] + MARCH_CANDIDATES
+BATCH_CANDIDATES = [c for c in MARCH_CANDIDATES if c["chunks"] == 0]
 
 
 def workload_key(volume_dims, nlevels, width, height, params, device_name=None) -> str:
@@ -117,7 +118,8 @@ def tune_batch(ctx, make_batch, key: str, candidates=None, cache_path=None, reps
     if hit:
         ctx.set_tuning(**hit["tuning"])
         return make_batch(), hit["tuning"], hit.get("ms", {})
-    candidates = candidates or MARCH_CANDIDATES
+    # a batch takes one queue item per tile whatever the chunk setting
+    candidates = candidates or BATCH_CANDIDATES
 
     def measure(cand):
         b = make_batch()

@@ -439,6 +439,7 @@ extern "C" vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning*
     VX_REQUIRE(t->tile >= 0 && t->tile <= 2, "tuning.tile %d outside 0..2", t->tile);
     VX_REQUIRE(t->order == 0 || t->order == 1, "tuning.order %d not 0/1", t->order);
     VX_REQUIRE(t->lights == 0 || t->lights == 1, "tuning.lights %d not 0/1", t->lights);
+    VX_REQUIRE(t->chunks >= 0 && t->chunks <= 64, "tuning.chunks %d outside 0..64", t->chunks);
     ctx->tuning = *t;
     return VX_OK;
 }
@@ -1363,10 +1364,34 @@ static vx_status march_shape(vx_ctx* ctx, bool single, MarchShape& sh) {
 
 static void march_common(MarchArgs& A, const vx_trace_params& p, const vx_trace_tuning& tu) {
     A.nrays = p.n_shadow + p.n_ao + p.n_spec;
+    // chunks (0: set_bands picks) | shadow rays << 8 | spec rays << 16
+    A.gen = (uint32_t)std::max(tu.chunks, 0) | ((uint32_t)p.n_shadow << 8) | ((uint32_t)p.n_spec << 16);
     A.refill = (tu.refill >= 1 && tu.refill <= 64) ? tu.refill : 16;
+
     A.maxt[0] = p.shadow_maxt;
     A.maxt[1] = p.ao_maxt;
     A.maxt[2] = p.spec_maxt;
+}
+
+// The tile queue's eight XCD bands over A.ntiles tiles: each band holds a
+// contiguous range of tiles, once per chunk of their rays. Chunks (queue
+// items per tile) unless tuned: a tile's rays are a few hundred microseconds
+// of a wave's work and the launch ends when its last wave finishes its last
+// item, so a launch takes chunks of about six rays, and more (down to one ray)
+// when that leaves fewer than two items per wave of the grid. The band
+// counters stay below 2^31.
+static void set_bands(MarchArgs& A, uint32_t waves) {
+    A.band_tiles = cdiv(A.ntiles, 8);
+    uint64_t want = A.gen & 0xffu;
+    if (want == 0) {
+        const uint64_t spread = cdiv(2ull * waves, std::max<uint64_t>(A.ntiles, 1));
+        want = std::max<uint64_t>((uint64_t)A.nrays / 6, spread);
+    }
+    const uint64_t cap = (0x7fffffffull) / (8ull * std::max<uint64_t>(A.band_tiles, 1));
+    const uint32_t chunks = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(std::min<uint64_t>(want, std::max<int32_t>(A.nrays, 1)), cap));
+    A.band = A.band_tiles * chunks;
+    A.gen = (A.gen & ~0xffu) | chunks;
 }
 
 // Scratch of one single-view march launch: per-slot results, listed rays,
@@ -1392,7 +1417,6 @@ static vx_status enqueue_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tr
     A.items = nullptr;
     A.nitems = 1;
     A.ntiles = ntiles;
-    A.band = cdiv(A.ntiles, 8);
     const size_t slots = (size_t)A.ntiles * (size_t)A.nrays * 64;
     A.counters = M.counters;
     A.deferred = M.deferred;
@@ -1402,6 +1426,7 @@ static vx_status enqueue_trace_march(vx_ctx* ctx, const vx_volume* vol, const Tr
     A.hits = hits;
     MarchShape sh;
     if (vx_status st = march_shape(ctx, true, sh)) return st;
+    set_bands(A, sh.grid * (uint32_t)(sh.block / 64));
     hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(kCounterWords, 256)), dim3(256), 0, stream, A.counters, (int)kCounterWords);
     hipError_t e = launch_march(sh.block, sh.steps, true, sh.grid, sh.rest_grid, stream, A, e0, e1);
     if (e == hipSuccess) e = launch_status();
@@ -1908,13 +1933,15 @@ static vx_status batch_plan(vx_batch* b, const vx_batch_item* items, uint32_t n)
     A.tile_item = b->tile_item;
     A.nitems = n;
     A.ntiles = (uint32_t)tiles;
-    A.band = cdiv(tiles, 8);
     A.counters = b->march.counters;
     A.deferred = b->march.deferred;
     A.deferred_cap = (uint32_t)slots;
     A.res = b->march.res;
     A.fres = fetches ? b->march.fres : nullptr;
     if (vx_status st = march_shape(ctx, false, b->shape)) return st;
+    // a batch takes one queue item per tile (its launches hold many tiles)
+    A.gen = (A.gen & ~0xffu) | 1u;
+    set_bands(A, b->shape.grid * (uint32_t)(b->shape.block / 64));
     // capture the launch; vx_batch_launch replays it
     if (!b->cap) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);

@@ -65,7 +65,9 @@ struct MarchArgs {
     const MarchItem* items;    // device array of a batched launch
     const uint16_t* tile_item; // item of each global tile (batched launches; null: search)
     uint32_t nitems, ntiles;   // items, global tiles
-    uint32_t band;             // tiles per XCD band
+    uint32_t band_tiles;       // tiles per XCD band
+    uint32_t band;             // queue items per XCD band: band_tiles x chunks
+    uint32_t gen;              // chunks (queue items per tile) | n_shadow << 8 | n_spec << 16
     uint32_t* counters;        // band counters, listed-ray count (kCounterStride apart)
     uint32_t* deferred;        // slots of rays left to the literal path
     uint32_t deferred_cap;     // capacity of deferred (every slot fits)

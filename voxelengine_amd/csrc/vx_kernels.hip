@@ -967,6 +967,13 @@ VXD void advance_tmax(MarchRay& R, uint64_t e0, uint64_t e1, uint64_t e2, uint64
         : "scc");
 }
 
+// The pointer as a value in vector registers (the same value in every lane).
+VXD const uint64_t* in_vgpr(const uint64_t* p) {
+    uint64_t v = reinterpret_cast<uint64_t>(p);
+    asm volatile("" : "+v"(v));
+    return reinterpret_cast<const uint64_t*>(v);
+}
+
 // meta += inc in the lanes of m (exec restored before the block ends).
 VXD void add_masked(uint32_t& meta, uint32_t inc, uint64_t m) {
     uint64_t sv;
@@ -997,11 +1004,12 @@ k_march(MarchArgs A) {
     const int32_t nrays = A.nrays;
     const uint64_t lane_lt = (1ull << lane) - 1ull;
 
-    // generation cursor (wave-uniform)
+    // generation cursor (wave-uniform): queue item = (tile, chunk of its
+    // rays); gr runs over the chunk's generation indices [gr, gend)
     uint32_t q = blockIdx.x & 7u, misses = 0;
     bool gen_done = nrays <= 0;   // no rays per pixel: k_resolve writes the defaults
     uint32_t gitem = 0, gtile = 0;
-    int32_t gr = nrays;
+    int32_t gr = SINGLE ? 0 : nrays, gend = 0;
     uint32_t head = 0, tail = 0;
     // this lane's pixel of the generation tile
     PixState P;
@@ -1019,7 +1027,10 @@ k_march(MarchArgs A) {
     // the wave's item (its rays in flight all belong to it)
     uint32_t citem = SINGLE ? 0u : ~0u, nitem = 0;
     bool pending = false;
-    const uint64_t* grid = A.item0.grid;
+    // the item's grid address, held in vector registers: the steps use it for
+    // every fetch, and in scalar registers it competes with the launch's
+    // uniform state (the compiler then re-reads it from a spill every step)
+    const uint64_t* grid = SINGLE ? in_vgpr(A.item0.grid) : A.item0.grid;
     uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy;
 
     while (true) {
@@ -1081,22 +1092,39 @@ k_march(MarchArgs A) {
             // ---- generate until the ring holds 64 rays or the work is out
             while (tail - head < 64u && !gen_done) {
                 f3 wd = mk3(0.0f, 0.0f, 0.0f);
-                if (gr >= nrays) {
+                if (gr >= (SINGLE ? gend : nrays)) {
                     if (!pending) {
-                        // next tile: own band first, then the others
-                        uint32_t tile = 0;
+                        // next queue item: own band first, then the others. A
+                        // band is its tile range once per chunk; the last band's
+                        // items past the last tile are skipped.
+                        uint32_t tile = 0, chunk = 0;
                         bool got = false;
                         while (misses < 8u) {
                             uint32_t t = 0;
                             if (lane == 0) t = atomicAdd(&A.counters[q * kCounterStride], 1u);
                             t = __builtin_amdgcn_readfirstlane(t);
-                            tile = q * A.band + t;
-                            if (t < A.band && tile < A.ntiles) { got = true; break; }
+                            if (SINGLE) {
+                                if (t < A.band) {
+                                    chunk = t / A.band_tiles;
+                                    tile = q * A.band_tiles + (t - chunk * A.band_tiles);
+                                    if (tile < A.ntiles) { got = true; break; }
+                                    continue;
+                                }
+                            } else {
+                                // a batch takes one item per tile
+                                tile = q * A.band + t;
+                                if (t < A.band && tile < A.ntiles) { got = true; break; }
+                            }
                             q = (q + 1u) & 7u;
                             misses++;
                         }
                         if (!got) { gen_done = true; break; }
                         gtile = tile;
+                        if (SINGLE) {
+                            const uint32_t chunks = A.gen & 0xffu;
+                            gr = (int32_t)((chunk * (uint32_t)nrays) / chunks);
+                            gend = (int32_t)(((chunk + 1u) * (uint32_t)nrays) / chunks);
+                        }
                         nitem = __builtin_amdgcn_readfirstlane(item_of_tile<SINGLE>(A, gtile));
                     }
                     if (!SINGLE && nitem != citem) {
@@ -1117,18 +1145,25 @@ k_march(MarchArgs A) {
                     uint32_t i = 0;
                     pvalid = tile_lane_pixel(I, gtile - I.tile0, lane, i) && I.T.mat[i] != 0;
                     if (pvalid) pixel_setup(I.T, i, P, wd);
-                    gr = 0;
+                    if (!SINGLE) gr = 0;
                 }
-                // one batch: ray gr of every valid pixel of the tile
+                // one batch: ray r of every valid pixel of the tile; generation
+                // index gr runs shadow rays, then spec rays (the long ones,
+                // early in the launch), then AO rays
+                int32_t r = gr;
+                if (SINGLE) {
+                    const int32_t nsh = (int32_t)((A.gen >> 8) & 0xffu), nsp = (int32_t)(A.gen >> 16);
+                    r = gr < nsh ? gr : (gr < nsh + nsp ? nrays - 1 - (gr - nsh) : gr - nsp);
+                }
                 const MarchItem& I = march_item_uniform<SINGLE>(A, gitem);
                 f3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(1.0f, 1.0f, 1.0f);
                 float maxt = 1.0f;
                 int32_t kind = 0;
                 int32_t cv[3] = {0, 0, 0};
                 bool push = false;
-                const uint32_t slot = ((gtile * (uint32_t)nrays + (uint32_t)gr) << 6) | lane;
+                const uint32_t slot = ((gtile * (uint32_t)nrays + (uint32_t)r) << 6) | lane;
                 if (pvalid) {
-                    kind = pixel_ray(I.T, P, wd, gr, o, d, maxt);
+                    kind = pixel_ray(I.T, P, wd, r, o, d, maxt);
                     if (!fast_ray_ok(o, d)) {
                         uint32_t k = atomicAdd(&A.counters[8 * kCounterStride], 1u);
                         A.deferred[k] = slot;
