@@ -560,6 +560,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the headline trace (for profiling)")
+    ap.add_argument("--extras", default="all",
+                    help="comma list of the extra workloads to run: reference, backward, lights, reflection, "
+                         "batched, prefabs, large (default all)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed region, copy 1 GiB device-to-device (calibrates profiler byte counters)")
     args = ap.parse_args()
@@ -686,7 +689,10 @@ def main():
         "primary_ms": round(primary_ms, 4),
     }
 
-    if not args.no_extras:
+    extras = set() if args.no_extras else (
+        {"reference", "backward", "lights", "reflection", "batched", "prefabs", "large"} if args.extras == "all"
+        else set(args.extras.split(",")))
+    if "reference" in extras:
         # same frame through the reference's enabled marchers (Sparse / SuperSparse / Sparse)
         p_ref = configs.trace_params(**dict(cfg["params"], march="reference"))
         for _ in range(2):
@@ -699,6 +705,7 @@ def main():
         torch.cuda.synchronize()
         out["reference_marchers_grays_per_s"] = round(rays_per_frame / (ev0.elapsed_time(ev1) / 10 * 1e-3) / 1e9, 4)
 
+    if "backward" in extras:
         # backward: shading adjoint over the same frame (visibility fixed)
         light = configs.default_light()
         rgb = torch.zeros((W * H, 3), dtype=torch.float32, device="cuda")
@@ -718,13 +725,17 @@ def main():
         out["backward"] = {"ms": round(bwd_ms, 4), "grays_per_s": round(rays_per_frame / (bwd_ms * 1e-3) / 1e9, 2),
                            "px_per_s": round(hit_px / (bwd_ms * 1e-3), 1)}
 
-    if not args.no_extras:
+    if "lights" in extras:
         log("light pass ...")
         out["lights"] = bench_lights(ctx, occ, vd, g, pal_d, cfg["dims"])
+    if "reflection" in extras:
         out.update(bench_reflection_ssao(ctx, occ, vd, g, pal_d))
+    if "batched" in extras:
         out["batched"] = bench_batched(ctx, pal_d)
+    if "prefabs" in extras:
         log("prefabs ...")
         out["prefabs"] = bench_prefabs(ctx)
+    if "large" in extras:
         log("config L ...")
         out["large_batched"] = bench_large(ctx, pal_d)
 
