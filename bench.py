@@ -38,6 +38,13 @@ KERNEL_SRC = os.path.join(ROOT, "voxelengine_amd", "csrc", "vx_kernels.hip")
 COUNTERS = os.path.join(ROOT, "profiles", "k_march_F_1080p_512_counters.json")
 MI355X_SIMDS = 1024          # 256 CUs x 4 SIMDs
 MI355X_CLOCK_HZ = 2.4e9      # peak engine clock
+# Issue ceilings in wave instructions per SIMD per cycle, measured on the
+# device by voxelengine_amd/valu_peak (independent chains of v_add_f32 /
+# v_add_u32 / s_add_u32, 8 waves per SIMD; rocprofv3 SQ_INSTS_* over
+# GRBM_GUI_ACTIVE cycles): a wave64 vector instruction every 2.25 cycles, a
+# scalar one every 4.1.
+VALU_ISSUE_PER_CYCLE = 0.444
+SALU_ISSUE_PER_CYCLE = 0.244
 
 
 def kernel_src_sha16():
@@ -93,9 +100,10 @@ def limiter(counters, kernel_ms):
     """What the rocprofv3 counters of this kernel source show bounds k_march,
     each as a fraction of the counted launch's time (1.0 = that resource busy
     for the whole launch): vector and scalar instruction issue (SQ_INSTS_VALU /
-    SQ_INSTS_SALU wave instructions, one per SIMD every 4 cycles, over 1024
-    SIMDs at the measured engine clock), the vector-memory address and data
-    paths (TA / TD busy cycles per CU), waves waiting on memory counters
+    SQ_INSTS_SALU wave instructions over 1024 SIMDs at the measured engine
+    clock, against the issue ceilings measured on the device), the
+    vector-memory address and data paths (TA / TD busy cycles per CU; busy
+    includes waiting for data), waves waiting on memory counters
     (SQ_WAIT_ANY / SQ_WAVE_CYCLES) and measured HBM traffic against the HBM
     roofline. `bound` names the largest. None without counters."""
     if not counters:
@@ -105,8 +113,8 @@ def limiter(counters, kernel_ms):
     t = (counters.get("kernel_ns_in_ta_pass") or kernel_ms * 1e6) * 1e-9
     clock = counters.get("engine_clock_hz") or MI355X_CLOCK_HZ
     cyc = t * clock
-    fr = {"valu_issue": counters["sq_insts_valu_per_launch"] * 4.0 / MI355X_SIMDS / cyc,
-          "salu_issue": counters["sq_insts_salu_per_launch"] * 4.0 / MI355X_SIMDS / cyc,
+    fr = {"valu_issue": counters["sq_insts_valu_per_launch"] / VALU_ISSUE_PER_CYCLE / MI355X_SIMDS / cyc,
+          "salu_issue": counters["sq_insts_salu_per_launch"] / SALU_ISSUE_PER_CYCLE / MI355X_SIMDS / cyc,
           "hbm": counters["traffic_bytes_per_launch"] / t / (HBM_PEAK_GBS * 1e9)}
     if "ta_busy_per_cu" in counters:
         fr["ta_busy"] = counters["ta_busy_per_cu"] / cyc

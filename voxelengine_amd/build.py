@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvx_amd.so")
+PROBE = os.path.join(HERE, "valu_peak")
 SOURCES = ["vx_kernels.hip", "vx_api.hip", "vx_assets.cpp"]
 ARCH = "gfx950"
 
@@ -54,6 +55,11 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # the vector-issue probe whose rate the trace's counters summary uses
+    cmd = [hipcc(), *[f for f in HIP_FLAGS if f != "-fPIC"], os.path.join(CSRC, "valu_peak.hip"), "-o", PROBE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
     return LIB
 
 
