@@ -208,7 +208,8 @@ struct vx_ctx {
     uint32_t* lt_total;
     size_t lt_px_cap;
     LightRec* lt_rec;
-    float* lt_term;
+    float* lt_shadow;
+    uint16_t* lt_slot_light;
     size_t lt_ray_cap;
     uint16_t* lt_rfetch;
     size_t lt_fetch_cap;
@@ -379,7 +380,8 @@ extern "C" vx_status vx_ctx_create(int32_t device, vx_ctx** out) {
     c->lt_first = c->lt_count = c->lt_total = nullptr;
     c->lt_px_cap = 0;
     c->lt_rec = nullptr;
-    c->lt_term = nullptr;
+    c->lt_shadow = nullptr;
+    c->lt_slot_light = nullptr;
     c->lt_ray_cap = 0;
     c->lt_rfetch = nullptr;
     c->lt_fetch_cap = 0;
@@ -415,7 +417,8 @@ extern "C" vx_status vx_ctx_destroy(vx_ctx* ctx) {
     (void)hipFree(ctx->lt_count);
     (void)hipFree(ctx->lt_total);
     (void)hipFree(ctx->lt_rec);
-    (void)hipFree(ctx->lt_term);
+    (void)hipFree(ctx->lt_shadow);
+    (void)hipFree(ctx->lt_slot_light);
     (void)hipFree(ctx->lt_rfetch);
     (void)hipFree(ctx->rb_aabb);
     (void)hipFree(ctx->rb_ops);
@@ -1590,7 +1593,8 @@ static vx_status light_pass_list(vx_ctx* ctx, const vx_volume* vol, const LightA
     B.first = ctx->lt_first;
     B.count = ctx->lt_count;
     B.rec = nullptr;
-    B.term = nullptr;
+    B.shadow = nullptr;
+    B.slot_light = nullptr;
     B.rfetch = nullptr;
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, B.total, 1);
     hipLaunchKernelGGL(k_light_count, dim3(8 * A.tiles.per_xcd), dim3(256), 0, stream, A, B);
@@ -1606,13 +1610,16 @@ static vx_status light_pass_list(vx_ctx* ctx, const vx_volume* vol, const LightA
     size_t rc = ctx->lt_ray_cap;
     if (vx_status st = grow(ctx->lt_rec, rc, total)) return st;
     rc = ctx->lt_ray_cap;
-    if (vx_status st = grow(ctx->lt_term, rc, 3 * (size_t)total)) return st;
+    if (vx_status st = grow(ctx->lt_shadow, rc, total)) return st;
+    rc = ctx->lt_ray_cap;
+    if (vx_status st = grow(ctx->lt_slot_light, rc, total)) return st;
     ctx->lt_ray_cap = std::max<size_t>(ctx->lt_ray_cap, total);
     if (A.fetches) {
         if (vx_status st = grow(ctx->lt_rfetch, ctx->lt_fetch_cap, total)) return st;
     }
     B.rec = ctx->lt_rec;
-    B.term = ctx->lt_term;
+    B.shadow = ctx->lt_shadow;
+    B.slot_light = ctx->lt_slot_light;
     B.rfetch = A.fetches ? ctx->lt_rfetch : nullptr;
     hipLaunchKernelGGL(k_light_records, dim3(8 * A.tiles.per_xcd), dim3(256), 0, stream, A, B);
     VX_LAUNCHED("k_light_records");

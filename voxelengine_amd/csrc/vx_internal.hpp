@@ -131,10 +131,15 @@ struct DepthArgs {
 };
 
 // The light pass's ray list (k_light_count / _records / _march / _fold).
+// A listed shadow ray: origin, direction and length from the light's draw
+// (light_ray), its marcher, and its slot (its pixel's first slot + the
+// light's rank among the pixel's lights).
 struct LightRec {
-    uint32_t pixel;
-    uint32_t light;
-    uint32_t slot;             // the ray's term slot: its pixel's first record + the light's rank
+    float o[3];
+    float d[3];
+    float hit_dist;
+    int32_t march;
+    uint32_t slot;
 };
 
 struct LightListArgs {
@@ -142,7 +147,8 @@ struct LightListArgs {
     uint32_t* first;           // per pixel: its first record
     uint32_t* count;           // per pixel: its in-range lights
     LightRec* rec;             // the records, per tile light by light (pixels in order)
-    float* term;               // 3 per slot: the light's shadowed direct term
+    float* shadow;             // per slot: the march's shadow factor (0 or 1)
+    uint16_t* slot_light;      // per slot: the light's index
     uint16_t* rfetch;          // per slot: the marcher's fetch count, or null
 };
 

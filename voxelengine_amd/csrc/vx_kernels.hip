@@ -1668,13 +1668,13 @@ __global__ void __launch_bounds__(256) k_lights(OccView V, LightArgs A) {
 // listed, marched one per lane, and their terms folded per pixel in light
 // order. Pass 1 counts each pixel's in-range lights and gives each 16x16
 // tile a contiguous range of the list (one atomic per tile) and each pixel
-// a contiguous range of term slots in it; pass 2 writes the tile's records
-// light by light, each light's in-range pixels in pixel order, so that the
-// 64 rays a wave marches run from neighbouring pixels to one light (similar
-// directions and lengths); pass 3 marches every record and stores its term in
-// its slot; pass 4 adds a pixel's terms to its light buffer value in slot
-// order, which is draw order. The arithmetic is k_lights' (the same device
-// functions).
+// a contiguous range of slots in it; pass 2 sets each pixel up once and
+// writes the tile's rays (origin, direction, length, marcher, slot) light by
+// light, each light's in-range pixels in pixel order, so that neighbouring
+// rays run from neighbouring pixels to one light; pass 3 marches every ray
+// and stores its shadow factor in its slot; pass 4 sets each pixel up again
+// and adds its lights' terms to its light buffer value in slot order, which
+// is draw order. The arithmetic is k_lights' (the same device functions).
 // ---------------------------------------------------------------------------
 template <bool WRITE>
 VXD void light_list_tile(const LightArgs& A, const LightListArgs& B) {
@@ -1729,22 +1729,32 @@ VXD void light_list_tile(const LightArgs& A, const LightListArgs& B) {
         const uint64_t lane_lt = (1ull << lane) - 1ull;
         if (threadIdx.x == 0) tile_base_s = B.first[i];
         uint32_t slot = valid ? B.first[i] : 0u;
+        LightPix P;
+        if (valid) light_pixel_setup(A, i, P);
         __syncthreads();
         uint32_t off = tile_base_s;
         for (uint32_t k = 0; k < n; ++k) {
             const uint32_t l = list_s[k];
+            const vx_light& L = A.lights[__builtin_amdgcn_readfirstlane(l)];
             f3 ld;
             float dist;
-            const bool in = valid && light_in_range(A.lights[__builtin_amdgcn_readfirstlane(l)], wpos, ld, dist);
+            const bool in = valid && light_in_range(L, wpos, ld, dist);
             const uint64_t m = __ballot(in);
             if (lane == 0) wsum_s[wave] = (uint32_t)__popcll(m);
             __syncthreads();
             uint32_t at = off + (uint32_t)__popcll(m & lane_lt);
             for (uint32_t w = 0; w < wave; ++w) at += wsum_s[w];
             if (in) {
-                B.rec[at].pixel = (uint32_t)i;
-                B.rec[at].light = l;
-                B.rec[at].slot = slot;
+                f3 o, wdn;
+                float hit_dist;
+                light_ray(L, P, ld, dist, o, wdn, hit_dist);
+                LightRec& r = B.rec[at];
+                r.o[0] = o.x; r.o[1] = o.y; r.o[2] = o.z;
+                r.d[0] = wdn.x; r.d[1] = wdn.y; r.d[2] = wdn.z;
+                r.hit_dist = hit_dist;
+                r.march = L.type == VX_LIGHT_SPOT ? A.params.spot_march : A.params.point_march;
+                r.slot = slot;
+                B.slot_light[slot] = (uint16_t)l;
                 ++slot;
             }
             off += (wsum_s[0] + wsum_s[1]) + (wsum_s[2] + wsum_s[3]);
@@ -1756,29 +1766,17 @@ VXD void light_list_tile(const LightArgs& A, const LightListArgs& B) {
 __global__ void __launch_bounds__(256) k_light_count(LightArgs A, LightListArgs B) { light_list_tile<false>(A, B); }
 __global__ void __launch_bounds__(256) k_light_records(LightArgs A, LightListArgs B) { light_list_tile<true>(A, B); }
 
-// One shadow ray per thread over the list (grid-stride to the device total).
+// One shadow ray per thread over the list (grid-stride to the device total):
+// its shadow factor and fetch count into its slot.
 __global__ void __launch_bounds__(256) k_light_march(OccView V, LightArgs A, LightListArgs B) {
-    const uint32_t total = *B.total;
+    const uint32_t total = B.total[0];
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) {
         const LightRec r = B.rec[j];
-        LightPix P;
-        light_pixel_setup(A, r.pixel, P);
-        const vx_light& L = A.lights[r.light];
-        f3 ld;
-        float dist;
-        (void)light_in_range(L, P.wpos, ld, dist);
-        f3 o, wdn;
-        float hit_dist;
-        light_ray(L, P, ld, dist, o, wdn, hit_dist);
         uint32_t nf = 0;
         int32_t hit;
-        const float d =
-            march_dist(V, L.type == VX_LIGHT_SPOT ? A.params.spot_march : A.params.point_march, o, wdn, hit_dist, nf, hit);
-        float term[3];
-        light_term(L, P, ld, dist, (d < hit_dist) ? 0.0f : 1.0f, term);
-        B.term[3 * (size_t)r.slot + 0] = term[0];
-        B.term[3 * (size_t)r.slot + 1] = term[1];
-        B.term[3 * (size_t)r.slot + 2] = term[2];
+        const float d = march_dist(V, r.march, mk3(r.o[0], r.o[1], r.o[2]), mk3(r.d[0], r.d[1], r.d[2]), r.hit_dist,
+                                   nf, hit);
+        B.shadow[r.slot] = (d < r.hit_dist) ? 0.0f : 1.0f;
         if (B.rfetch) B.rfetch[r.slot] = (uint16_t)nf;
     }
 }
@@ -1794,11 +1792,18 @@ __global__ void __launch_bounds__(256) k_light_fold(LightArgs A, LightListArgs B
     const uint32_t j0 = B.first[i], n = B.count[i];
     float acc[3] = {A.rgb[3 * (size_t)i + 0], A.rgb[3 * (size_t)i + 1], A.rgb[3 * (size_t)i + 2]};
     uint32_t nf = 0;
+    LightPix P;
+    if (n) light_pixel_setup(A, i, P);
     for (uint32_t k = 0; k < n; ++k) {
-        const float* t = B.term + 3 * (size_t)(j0 + k);
-        acc[0] = acc[0] + t[0];
-        acc[1] = acc[1] + t[1];
-        acc[2] = acc[2] + t[2];
+        const vx_light& L = A.lights[B.slot_light[j0 + k]];
+        f3 ld;
+        float dist;
+        (void)light_in_range(L, P.wpos, ld, dist);
+        float term[3];
+        light_term(L, P, ld, dist, B.shadow[j0 + k], term);
+        acc[0] = acc[0] + term[0];
+        acc[1] = acc[1] + term[1];
+        acc[2] = acc[2] + term[2];
         if (A.fetches) nf += B.rfetch[j0 + k];
     }
     A.rgb[3 * (size_t)i + 0] = acc[0];
