@@ -168,7 +168,7 @@ typedef struct {
     int32_t steps;           /* DDA steps between refill checks: 2, 4 or 8 (0 = 8) */
     int32_t tile;            /* pixels of a ray-parallel tile: 0 = 8x8, 1 = 16x4, 2 = 4x16 */
     int32_t order;           /* tile queue order: 0 = rows of tiles, 1 = rows of 8x8-tile blocks */
-    int32_t lights;          /* vx_light_pass: 0 = list of (pixel, light) rays, one per lane; 1 = a loop per pixel */
+    int32_t lights;          /* vx_light_pass: 0 = a loop per pixel; 1 = list of (pixel, light) rays, one per lane */
     int32_t chunks;          /* ray-parallel queue items per tile of a single-view trace: its rays in this
                                 many chunks, 1..64 (0 = auto); a batch takes one item per tile */
 } vx_trace_tuning;

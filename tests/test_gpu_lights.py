@@ -30,7 +30,7 @@ def _lights(dims, n_point=8, n_spot=5, seed=4):
     return L
 
 
-@pytest.fixture(params=[0, 1], ids=["ray_list", "pixel_loop"])
+@pytest.fixture(params=[1, 0], ids=["ray_list", "pixel_loop"])
 def lights_path(request, ctx):
     ctx.set_tuning(lights=request.param)
     yield request.param

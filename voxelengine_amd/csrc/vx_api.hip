@@ -1652,7 +1652,7 @@ extern "C" vx_status vx_light_pass(vx_ctx* ctx, const vx_volume* vol, const vx_v
     A.params = *params;
     A.rgb = rgb;
     A.fetches = fetches;
-    if (ctx->tuning.lights == 1 || nlights == 0) {
+    if (ctx->tuning.lights == 0 || nlights == 0) {
         hipLaunchKernelGGL(k_lights, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), occ_view(vol), A);
         VX_LAUNCHED("k_lights");
         return VX_OK;

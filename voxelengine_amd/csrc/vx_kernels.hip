@@ -967,11 +967,17 @@ VXD void advance_tmax(MarchRay& R, uint64_t e0, uint64_t e1, uint64_t e2, uint64
         : "scc");
 }
 
-// The pointer as a value in vector registers (the same value in every lane).
-VXD const uint64_t* in_vgpr(const uint64_t* p) {
+// A march grid as a global-memory pointer (its fetches are global loads, not
+// flat ones), optionally held in vector registers (the same value in every
+// lane).
+typedef const __attribute__((address_space(1))) uint64_t* GridPtr;
+VXD GridPtr grid_ptr(const uint64_t* p) {
+    return (GridPtr)p;
+}
+VXD GridPtr in_vgpr(const uint64_t* p) {
     uint64_t v = reinterpret_cast<uint64_t>(p);
     asm volatile("" : "+v"(v));
-    return reinterpret_cast<const uint64_t*>(v);
+    return (GridPtr)v;
 }
 
 // meta += inc in the lanes of m (exec restored before the block ends).
@@ -1030,7 +1036,7 @@ k_march(MarchArgs A) {
     // the item's grid address, held in vector registers: the steps use it for
     // every fetch, and in scalar registers it competes with the launch's
     // uniform state (the compiler then re-reads it from a spill every step)
-    const uint64_t* grid = SINGLE ? in_vgpr(A.item0.grid) : A.item0.grid;
+    GridPtr grid = SINGLE ? in_vgpr(A.item0.grid) : grid_ptr(A.item0.grid);
     uint32_t ggx = (uint32_t)A.item0.gx, ggxy = (uint32_t)A.item0.gxy;
 
     while (true) {
@@ -1135,7 +1141,7 @@ k_march(MarchArgs A) {
                         }
                         citem = nitem;
                         const MarchItem& J = march_item_uniform<SINGLE>(A, citem);
-                        grid = J.grid;
+                        grid = grid_ptr(J.grid);
                         ggx = (uint32_t)J.gx;
                         ggxy = (uint32_t)J.gxy;
                     }
@@ -1252,12 +1258,14 @@ k_march(MarchArgs A) {
             // two or three axes step together (rare): best_t sums their t. (A
             // lane without a ray holds distinct t_max values, so it never
             // counts here.)
+            // (median == minimum: two vector instructions, where the masks'
+            // scalar logic took five scalar ones)
             float bt = tmin;
-            if (__builtin_expect(((m0 & m1) | (m2 & (m0 | m1))) != 0, 0)) {
-                const uint64_t tie = (m0 & m1) | (m2 & (m0 | m1));
-                if (tie & lane_bit) {
+            const bool tie = __builtin_amdgcn_fmed3f(R.tm0, R.tm1, R.tm2) == tmin;
+            if (__builtin_expect(__ballot(tie) != 0, 0)) {
+                if (tie) {
                     const float t2 = tmin + tmin;
-                    bt = (m0 & m1 & m2 & lane_bit) ? t2 + tmin : t2;
+                    bt = (R.tm0 == tmin && R.tm1 == tmin && R.tm2 == tmin) ? t2 + tmin : t2;
                 }
             }
             // A ray enters the march between blocks of STEPS steps with a
