@@ -201,26 +201,44 @@ VXD uint32_t texel_base(int32_t tx, int32_t ty, int32_t tz) {
 VXD uint32_t spread_byte(uint32_t b) {
     return (b & 3u) | ((b & 0xcu) << 2) | ((b & 0x30u) << 12) | ((b & 0xc0u) << 14);
 }
-// Reference byte of texel t (0 when outside [0, s)), texelFetch semantics.
-// Texels outside the volume read 0; the word is read unconditionally (cell
-// word 0 for those), so a march step carries no branch for it.
-VXD uint32_t texel_byte(const OccView& V, int32_t tx, int32_t ty, int32_t tz) {
-    const bool in = (uint32_t)tx < (uint32_t)V.sx && (uint32_t)ty < (uint32_t)V.sy && (uint32_t)tz < (uint32_t)V.sz;
-    // 32-bit index arithmetic unless the volume holds 2^32 cell words or more
-    const bool narrow = (uint64_t)(uint32_t)V.bx * (uint32_t)V.by * (uint32_t)V.bz < (1ull << 29);
-    size_t ci;
-    if (narrow)
-        ci = in ? (size_t)((((((uint32_t)(tz >> 2) * (uint32_t)V.by + (uint32_t)(ty >> 2)) * (uint32_t)V.bx +
-                             (uint32_t)(tx >> 2))
-                            << 3) +
-                           sub3(tx >> 1, ty >> 1, tz >> 1)))
-                : 0;
-    else
-        ci = in ? brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1) : 0;
-    const uint64_t w = V.cells[ci];
+// Whether the volume's cell words are indexed in 32 bits (fewer than 2^32
+// cell words: 2^29 bricks of 8).
+VXD bool volume_narrow(const OccView& V) {
+    return (uint64_t)(uint32_t)V.bx * (uint32_t)V.by * (uint32_t)V.bz < (1ull << 29);
+}
+// Cell-word index of texel t and whether t lies inside [0, s); texels
+// outside read cell word 0 (the word is read unconditionally, so a march step
+// carries no branch for it) and decode to 0. NARROW: volume_narrow(V).
+template <bool NARROW>
+VXD size_t texel_word_n(const OccView& V, int32_t tx, int32_t ty, int32_t tz, bool& in) {
+    in = (uint32_t)tx < (uint32_t)V.sx && (uint32_t)ty < (uint32_t)V.sy && (uint32_t)tz < (uint32_t)V.sz;
+    if (NARROW) {
+        uint32_t c = (((((uint32_t)(tz >> 2) * (uint32_t)V.by + (uint32_t)(ty >> 2)) * (uint32_t)V.bx +
+                              (uint32_t)(tx >> 2))
+                             << 3) +
+                            sub3(tx >> 1, ty >> 1, tz >> 1));
+        // computed for every lane, then selected (kept out of a branch)
+        asm volatile("" : "+v"(c));
+        return in ? c : 0u;
+    }
+    size_t c = brick_index(V, tx >> 2, ty >> 2, tz >> 2) * 8 + sub3(tx >> 1, ty >> 1, tz >> 1);
+    asm volatile("" : "+v"(c));
+    return in ? c : 0;
+}
+VXD size_t texel_word(const OccView& V, int32_t tx, int32_t ty, int32_t tz, bool& in) {
+    return volume_narrow(V) ? texel_word_n<true>(V, tx, ty, tz, in) : texel_word_n<false>(V, tx, ty, tz, in);
+}
+// Reference byte of texel t from its cell word w (texel_word's `in`).
+VXD uint32_t texel_from_word(uint64_t w, int32_t tx, int32_t ty, int32_t tz, bool in) {
     const uint32_t x = (uint32_t)(w >> texel_base(tx, ty, tz));
     const uint32_t b = (x & 3u) | ((x >> 2) & 0xcu) | ((x >> 12) & 0x30u) | ((x >> 14) & 0xc0u);
     return in ? b : 0u;
+}
+// Reference byte of texel t (0 when outside [0, s)), texelFetch semantics.
+VXD uint32_t texel_byte(const OccView& V, int32_t tx, int32_t ty, int32_t tz) {
+    bool in;
+    const size_t ci = texel_word(V, tx, ty, tz, in);
+    return texel_from_word(V.cells[ci], tx, ty, tz, in);
 }
 VXD bool level_bit(const OccView& V, int32_t lvl, int32_t cx, int32_t cy, int32_t cz) {
     size_t wi = ((size_t)(cz >> 2) * V.wy[lvl] + (cy >> 2)) * V.wx[lvl] + (cx >> 2);
