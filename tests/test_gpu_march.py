@@ -83,15 +83,19 @@ def test_march_volume_sizes_and_tunings(api, ctx, oracle, vol_dims):
 
 
 def test_step_limit_in_every_tuning(api, ctx, oracle):
-    """Spec rays through a 320^3 volume holding only a floor: many end at the
-    256-step limit (one ray per pixel, so such a pixel counts 4 * 256
-    fetches). Every tuning's block length, with and without fetch counts,
-    matches the oracle."""
+    """Spec rays through a 320^3 volume holding a floor (y < 2) and a ceiling
+    (y >= 150): a ray reflected off the floor needs more than 256 steps to
+    reach the ceiling (300 fine voxels up), so many end at the 256-step limit
+    (one ray per pixel, so such a pixel counts 4 * 256 fetches) and reads as a
+    miss (spec 1), where a march that ran past the limit would report the
+    ceiling's distance. Every tuning's block length, with and without fetch
+    counts, matches the oracle."""
     from voxelengine_amd import configs
     model = _model(api, ctx, (96, 80, 88), seed=11)
     S = (320, 320, 320)
     host = np.zeros(S[::-1], np.uint8)
     host[:, :2, :] = 9                       # a floor: y < 2
+    host[:, 150:, :] = 9                     # a ceiling: y >= 150
     occ = api.Volume(ctx, *S, nlevels=1)
     occ.upload_region(host, [(0, 0, 0, S[0], S[1], S[2])])
     params = configs.trace_params(n_shadow=0, n_ao=0, n_spec=1, spec_maxt=2000.0, march="dda")
@@ -101,7 +105,9 @@ def test_step_limit_in_every_tuning(api, ctx, oracle):
             for variant in (0, 2):
                 got, ref = _trace_both(api, ctx, oracle, occ, host, model, variant, params)
                 _assert_same(got, ref, f"tuning {tu} view {variant}")
-                assert (ref[3] == 1024).sum() > 50
+                limit = ref[3] == 1024
+                assert limit.sum() > 50
+                assert (ref[2][limit] == 1.0).all()
     finally:
         ctx.reset_tuning()
 
