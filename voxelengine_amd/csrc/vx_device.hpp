@@ -213,10 +213,15 @@ template <bool NARROW>
 VXD size_t texel_word_n(const OccView& V, int32_t tx, int32_t ty, int32_t tz, bool& in) {
     in = (uint32_t)tx < (uint32_t)V.sx && (uint32_t)ty < (uint32_t)V.sy && (uint32_t)tz < (uint32_t)V.sz;
     if (NARROW) {
-        uint32_t c = (((((uint32_t)(tz >> 2) * (uint32_t)V.by + (uint32_t)(ty >> 2)) * (uint32_t)V.bx +
-                              (uint32_t)(tx >> 2))
-                             << 3) +
-                            sub3(tx >> 1, ty >> 1, tz >> 1));
+        // brick (tx, ty, tz) >> 2 of a brick grid under 2^29 bricks of at most
+        // 1025 per axis: 24-bit multiplies are exact for a texel inside (and
+        // a texel outside reads word 0)
+        // (two v_mad_u32_u24: the compiler's own choice for the same sums is a
+        // quarter-rate v_mad_u64_u32)
+        uint32_t a, b;
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"((uint32_t)tz >> 2), "v"((uint32_t)V.by), "v"((uint32_t)ty >> 2));
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(b) : "v"(a), "v"((uint32_t)V.bx), "v"((uint32_t)tx >> 2));
+        uint32_t c = (b << 3) | sub3(tx >> 1, ty >> 1, tz >> 1);
         // computed for every lane, then selected (kept out of a branch)
         asm volatile("" : "+v"(c));
         return in ? c : 0u;

@@ -454,13 +454,11 @@ VXD bool sparse_batch(const OccView& V, f3& pos, f3 sd, float& dd, float sf, flo
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         // texel of the position: ivec3(pos / 2) while the step is fine,
-        // ivec3(pos) / 2 once it is coarse (getVolumeAt(ivec3(pos), 1))
-        int32_t tx, ty, tz;
-        if (COARSE) {
-            tx = f2i(p[k].x) / 2; ty = f2i(p[k].y) / 2; tz = f2i(p[k].z) / 2;
-        } else {
-            tx = f2i(p[k].x / 2.0f); ty = f2i(p[k].y / 2.0f); tz = f2i(p[k].z / 2.0f);
-        }
+        // ivec3(pos) / 2 once it is coarse (getVolumeAt(ivec3(pos), 1)). The
+        // two agree wherever either is a texel of the volume (truncating
+        // twice truncates once; past 2^31 both saturate outside), so both
+        // phases take ivec3(pos * 0.5).
+        const int32_t tx = f2i(p[k].x * 0.5f), ty = f2i(p[k].y * 0.5f), tz = f2i(p[k].z * 0.5f);
         const size_t ci = texel_word_n<NARROW>(V, tx, ty, tz, in[k]);
         w[k] = V.cells[ci];
         sh[k] = texel_base(tx, ty, tz);
