@@ -1905,6 +1905,23 @@ __global__ void __launch_bounds__(256) k_light_fold(LightArgs A, LightListArgs B
 // Primary hit: GeometryVoxel.frag (LOD 1, MIP_COUNT 2, TRANSPARENCY 1) for the
 // world model with an identity WorldMatrix.
 // ===========================================================================
+// Byte index (c2 * my + c1) * mx + c0 of cell c of a level in 32 bits, as
+// two v_mad_u32_u24 (the compiler's own form is three quarter-rate
+// v_mad_u64_u32): exact for a cell inside the level, the levels being at most
+// 4096 per axis (c2 * my + c1 < my * mz <= 2^24); a cell outside reads byte 0.
+VXD uint32_t level_index32(int32_t c0, int32_t c1, int32_t c2, uint32_t mx, uint32_t my) {
+    uint32_t a, r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(a) : "v"((uint32_t)c2), "v"(my), "v"((uint32_t)c1));
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(mx), "v"((uint32_t)c0));
+    return r;
+}
+// c < -1 || c > up on any axis, without branches: c + 1 as unsigned exceeds
+// up + 1 exactly then (c = INT_MAX wraps to 2^31).
+VXD bool level_oob(int32_t c0, int32_t c1, int32_t c2, const int32_t up[3]) {
+    return ((uint32_t)(c0 + 1) > (uint32_t)(up[0] + 1)) | ((uint32_t)(c1 + 1) > (uint32_t)(up[1] + 1)) |
+           ((uint32_t)(c2 + 1) > (uint32_t)(up[2] + 1));
+}
+
 // intersectVolume, GeometryVoxel.frag:63-125, in the model's voxel space.
 // res / frame: the view's resolution and frame (the glass dither).
 // The select products step(a,b)*step(a,c) are 0 or 1 and sign(d) is -1, 0 or
@@ -1971,10 +1988,10 @@ VXD int32_t intersect_volume_t(const ModelMips& A, const float res[2], int32_t f
                     c1 += b1 ? is[1] : 0;
                     c2 += b2 ? is[2] : 0;
                     cb[k][0] = c0; cb[k][1] = c1; cb[k][2] = c2;
-                    ob[k] = c0 < -1 || c0 > up[0] || c1 < -1 || c1 > up[1] || c2 < -1 || c2 > up[2];
+                    ob[k] = level_oob(c0, c1, c2, up);
                     ib[k] = (uint32_t)c0 < mx && (uint32_t)c1 < my && (uint32_t)c2 < mz;
                     if (narrow) {
-                        const uint32_t idx = (uint32_t)c0 + (uint32_t)c1 * mx + (uint32_t)c2 * (mx * my);
+                        const uint32_t idx = level_index32(c0, c1, c2, mx, my);
                         vb[k] = base[ib[k] ? idx : 0u];
                     } else {
                         const size_t idx = (size_t)(uint32_t)c0 + (size_t)(uint32_t)c1 * mx + (size_t)(uint32_t)c2 * mx * my;
@@ -2046,14 +2063,13 @@ VXD int32_t intersect_volume_t(const ModelMips& A, const float res[2], int32_t f
             cv[0] += b0 ? is[0] : 0;
             cv[1] += b1 ? is[1] : 0;
             cv[2] += b2 ? is[2] : 0;
-            const bool oob =
-                cv[0] < -1 || cv[0] > up[0] || cv[1] < -1 || cv[1] > up[1] || cv[2] < -1 || cv[2] > up[2];
+            const bool oob = level_oob(cv[0], cv[1], cv[2], up);
             // the texel is read unconditionally (byte 0 of the level for a cell
             // outside it, then taken as empty), so the step has one branch
             const bool inside = (uint32_t)cv[0] < mx && (uint32_t)cv[1] < my && (uint32_t)cv[2] < mz;
             uint32_t vox;
             if (narrow) {
-                const uint32_t idx = (uint32_t)cv[0] + (uint32_t)cv[1] * mx + (uint32_t)cv[2] * (mx * my);
+                const uint32_t idx = level_index32(cv[0], cv[1], cv[2], mx, my);
                 vox = base[inside ? idx : 0u];
             } else {
                 const size_t idx = (size_t)(uint32_t)cv[0] + (size_t)(uint32_t)cv[1] * mx + (size_t)(uint32_t)cv[2] * mx * my;
