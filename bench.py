@@ -8,7 +8,8 @@ the view; inputs (volume, levels, G-buffer) are resident in HBM before timing.
 
 Prints one JSON line. Extra fields: the reference-marcher variant of the same
 frame, the backward (shading adjoint) rate, rebuild and primary-hit times,
-the point/spot light pass (64 + 64 lights) over the same frame, config B
+the point/spot light pass (64 + 64 lights), the reflection pass and the
+screen-space occlusion over the same frame, config B
 (batched against plain launches), the prefabs (load, rebuild, one frame with
 geometry pass, trace and temporal accumulation), config L, the HBM roofline
 of the trace kernel and the CPU oracle baseline.
