@@ -93,7 +93,7 @@ def cpu_baseline(occ_host, vd, params, pos, nrm, mat, pal, budget_s=12.0):
     what = "the whole frame" if stride == 1 else f"every {stride}th hit pixel"
     return {"value": rays / t_total / 1e9, "unit": "Grays/s", "cores": cores, "kind": "port",
             "label": "C++ restatement (not lavapipe): the oracle, C with OpenMP over pixels",
-            "cpu_model": cpu_model(),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{what} ({sample.size} px) x {reps}: {rays} rays in {t_total:.1f} s"}
 
 
