@@ -165,7 +165,7 @@ typedef struct {
     int32_t block;           /* ray-parallel kernel threads per block: 256 or 512 (0 = 256) */
     int32_t blocks_per_cu;   /* persistent grid = CUs x this (0 = occupancy limit) */
     int32_t refill;          /* lanes without a running ray that trigger a refill, 1..64 (0 = 16) */
-    int32_t steps;           /* DDA steps between refill checks: 2, 4 or 8 (0 = 8) */
+    int32_t steps;           /* DDA steps between refill checks: 2, 4, 8 or 16 (0 = 16) */
     int32_t tile;            /* pixels of a ray-parallel tile: 0 = 8x8, 1 = 16x4, 2 = 4x16 */
     int32_t order;           /* tile queue order: 0 = rows of tiles, 1 = rows of 8x8-tile blocks */
     int32_t lights;          /* vx_light_pass: 0 = a loop per pixel; 1 = list of (pixel, light) rays, one per lane */

@@ -16,7 +16,8 @@ TUNINGS = [dict(kernel=2),
            dict(kernel=2, refill=64, steps=8, tile=1, order=1),
            dict(kernel=2, refill=7, steps=4, blocks_per_cu=1, order=1),
            dict(kernel=2, chunks=1),
-           dict(kernel=2, chunks=4, tile=2)]
+           dict(kernel=2, chunks=4, tile=2),
+           dict(kernel=2, block=512, steps=16, refill=32)]
 
 
 def oracle_hits(oracle, occ_h, vd, params, pos, nrm, mat, pal):

@@ -18,7 +18,8 @@ TUNINGS = [dict(kernel=2),
            dict(kernel=2, tile=0, order=1),
            dict(kernel=2, chunks=1),
            dict(kernel=2, chunks=5, tile=1),
-           dict(kernel=2, chunks=64, order=1, refill=3)]
+           dict(kernel=2, chunks=64, order=1, refill=3),
+           dict(kernel=2, block=512, steps=16, chunks=2)]
 
 
 @pytest.fixture(scope="module")

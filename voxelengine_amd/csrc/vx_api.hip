@@ -179,7 +179,7 @@ static hipError_t noalloc_scratch(int device, uint8_t** out) {
 // CU count and the kernel's occupancy do not change between launches.
 struct GridCache {
     int cus;
-    uint32_t grid[2][2][3];   // [single][block 256/512][steps 2/4/8]: 0 = not yet asked
+    uint32_t grid[2][2][4];   // [single][block 256/512][steps 2/4/8/16]: 0 = not yet asked
 };
 
 struct vx_ctx {
@@ -437,8 +437,8 @@ extern "C" vx_status vx_ctx_set_trace_tuning(vx_ctx* ctx, const vx_trace_tuning*
     VX_REQUIRE(t->block == 0 || t->block == 256 || t->block == 512, "tuning.block %d not 256/512", t->block);
     VX_REQUIRE(t->blocks_per_cu >= 0 && t->blocks_per_cu <= 64, "tuning.blocks_per_cu outside 0..64");
     VX_REQUIRE(t->refill >= 0 && t->refill <= 64, "tuning.refill outside 0..64");
-    VX_REQUIRE(t->steps == 0 || t->steps == 2 || t->steps == 4 || t->steps == 8, "tuning.steps %d not 2/4/8",
-               t->steps);
+    VX_REQUIRE(t->steps == 0 || t->steps == 2 || t->steps == 4 || t->steps == 8 || t->steps == 16,
+               "tuning.steps %d not 2/4/8/16", t->steps);
     VX_REQUIRE(t->tile >= 0 && t->tile <= 2, "tuning.tile %d outside 0..2", t->tile);
     VX_REQUIRE(t->order == 0 || t->order == 1, "tuning.order %d not 0/1", t->order);
     VX_REQUIRE(t->lights == 0 || t->lights == 1, "tuning.lights %d not 0/1", t->lights);
@@ -1283,7 +1283,8 @@ extern "C" vx_status vx_light_taa(vx_ctx* ctx, const vx_view* view, const vx_taa
 // want_per_cu when that is set), a multiple of 8 so every XCD band has blocks.
 // The occupancy query runs once per kernel and context.
 static vx_status persistent_grid(vx_ctx* ctx, bool single, int block, int steps, int want_per_cu, uint32_t& grid) {
-    uint32_t& per_cu_c = ctx->gcache.grid[single ? 1 : 0][block == 512 ? 1 : 0][steps == 2 ? 0 : (steps == 4 ? 1 : 2)];
+    uint32_t& per_cu_c = ctx->gcache.grid[single ? 1 : 0][block == 512 ? 1 : 0]
+                                                  [steps == 2 ? 0 : (steps == 4 ? 1 : (steps == 8 ? 2 : 3))];
     if (per_cu_c == 0) {
         int per_cu = 0;
         VX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_kernel(block, steps, single), block, 0));
@@ -1360,7 +1361,7 @@ struct MarchShape {
 static vx_status march_shape(vx_ctx* ctx, bool single, MarchShape& sh) {
     const vx_trace_tuning& tu = ctx->tuning;
     sh.block = tu.block == 512 ? 512 : 256;
-    sh.steps = tu.steps ? tu.steps : 8;
+    sh.steps = tu.steps ? tu.steps : 16;
     sh.rest_grid = 64;
     return persistent_grid(ctx, single, sh.block, sh.steps, tu.blocks_per_cu, sh.grid);
 }

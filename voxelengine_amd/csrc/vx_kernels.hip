@@ -1482,10 +1482,12 @@ const void* march_kernel(int block, int steps, bool single) {
     if (block == 256) {
         if (steps == 2) return march_kernel_t<256, 2>(single);
         if (steps == 8) return march_kernel_t<256, 8>(single);
+        if (steps == 16) return march_kernel_t<256, 16>(single);
         return march_kernel_t<256, 4>(single);
     }
     if (steps == 2) return march_kernel_t<512, 2>(single);
     if (steps == 8) return march_kernel_t<512, 8>(single);
+    if (steps == 16) return march_kernel_t<512, 16>(single);
     return march_kernel_t<512, 4>(single);
 }
 
@@ -1498,10 +1500,12 @@ hipError_t launch_march(int block, int steps, bool single, uint32_t grid, uint32
     if (block == 256) {
         if (steps == 2) launch_march_t<256, 2>(single, grid, s, A);
         else if (steps == 8) launch_march_t<256, 8>(single, grid, s, A);
+        else if (steps == 16) launch_march_t<256, 16>(single, grid, s, A);
         else launch_march_t<256, 4>(single, grid, s, A);
     } else {
         if (steps == 2) launch_march_t<512, 2>(single, grid, s, A);
         else if (steps == 8) launch_march_t<512, 8>(single, grid, s, A);
+        else if (steps == 16) launch_march_t<512, 16>(single, grid, s, A);
         else launch_march_t<512, 4>(single, grid, s, A);
     }
     hipError_t e = hipGetLastError();
