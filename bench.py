@@ -502,7 +502,9 @@ def summarize_profiles(d):
     read for the 1 GiB device copy of --copy-probe; the engine clock is
     GRBM_GUI_ACTIVE (summed over the 8 XCDs) over the launch's own duration
     in the same pass. Writes the counters file and returns it."""
-    march = "k_march<256, 8, true, false>"
+    # the single-view march of the default launch shape (256 threads, 16
+    # steps between refill checks), without fetch counts
+    march = "k_march<256, 16, true, false>"
 
     def per_launch(sub, names):
         rows = _counter_rows(os.path.join(d, sub, "run_counter_collection.csv"))
