@@ -20,6 +20,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -36,6 +37,8 @@ def log(*a):
 
 
 KERNEL_SRC = os.path.join(ROOT, "voxelengine_amd", "csrc", "vx_kernels.hip")
+# the bench re-tunes every run (force=True); the cache only records the choice
+TUNE_CACHE = os.path.join(tempfile.gettempdir(), "vx_bench_tune.json")
 COUNTERS = os.path.join(ROOT, "profiles", "k_march_F_1080p_512_counters.json")
 MI355X_SIMDS = 1024          # 256 CUs x 4 SIMDs
 MI355X_CLOCK_HZ = 2.4e9      # peak engine clock
@@ -173,7 +176,7 @@ def bench_batched(ctx, pal_d, reps=5):
 
     # the autotuner picks the launch shape of each side on this workload:
     # one tuning for the 64 plain launches, one for the batched graph
-    cache = os.path.join(ROOT, "gpurun_out", "tune_cache.json")
+    cache = TUNE_CACHE
     tuned_plain, plain_ms = autotune.tune(ctx, plain, "bench-B-plain", cache_path=cache, reps=2, force=True)
     batch, tuned_batch, batch_ms = autotune.tune_batch(
         ctx, lambda: api.Batch(ctx, [(it[1], it[2], it[3], bv) for it, bv in zip(items, bvis)], pal_d, params),
@@ -233,7 +236,7 @@ def bench_large(ctx, pal_d, reps=3):
     rays = hit * configs.rays_per_pixel(params)
     batch, tuned, tune_ms = autotune.tune_batch(
         ctx, lambda: api.Batch(ctx, items, pal_d, params), "bench-L", reps=1, force=True,
-        cache_path=os.path.join(ROOT, "gpurun_out", "tune_cache.json"),
+        cache_path=TUNE_CACHE,
         candidates=autotune.BATCH_CANDIDATES)
     batch.launch()
     torch.cuda.synchronize()
