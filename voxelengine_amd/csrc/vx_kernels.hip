@@ -1314,13 +1314,14 @@ k_march(MarchArgs A) {
         // ---- march: STEPS steps of the running lanes. The wave keeps which
         // lanes run as a lane mask; every lane evaluates the step and only the
         // masked lanes change. A running lane moves its position first, to
-        // the voxel the step enters when its own voxel is free, and fetches
-        // that voxel's grid word right away: the fetch is in flight while the
-        // step tests the current voxel (with the word fetched one step
-        // earlier), and the next step uses it. t_max advances only when the
-        // voxel was free and the ray goes on; a ray that ends keeps its
-        // t_max, and the finish steps its position back. A lane that does not
-        // run keeps its position (a lane without a ray sits at word 0).
+        // the voxel the step enters when its own voxel is free, tests the
+        // current voxel's bit in the word fetched one step earlier, and then
+        // fetches the entered voxel's word: the fetch is in flight through
+        // the rest of the step, and the next step uses it. t_max advances
+        // only when the voxel was free and the ray goes on; a ray that ends
+        // keeps its t_max, and the finish steps its position back. A lane
+        // that does not run keeps its position (a lane without a ray sits at
+        // word 0).
         __builtin_amdgcn_s_setprio(1);
         uint64_t run = __ballot(running);
         const uint64_t full = __builtin_amdgcn_read_exec();
@@ -1331,10 +1332,16 @@ k_march(MarchArgs A) {
             const float tmin = fminf(fminf(R.tm0, R.tm1), R.tm2);
             const uint64_t m0 = __ballot(R.tm0 == tmin), m1 = __ballot(R.tm1 == tmin), m2 = __ballot(R.tm2 == tmin);
             move_axes(R, m0, m1, m2, run, full);
-            // the word is fetched only when the position enters another cell
+            // the word is fetched only when the position enters another cell.
+            // The current voxel's bit is tested first (grid words are
+            // bit-reversed: voxel bit b sits at bit 63 - b), so the fetch
+            // lands in the register of the word it replaces; the empty asm
+            // keeps the test ahead of the fetch (otherwise the word is copied
+            // to a second register at every step to keep it for the test).
             const uint32_t ci = grid_word(R.x, R.y, R.z, ggx, ggxy);
-            uint64_t nw = w;
-            if (ci != R.ci) nw = grid[ci];
+            uint64_t occm = __ballot((int64_t)(w << b) < 0);
+            asm volatile("" : "+s"(occm) : : "memory");
+            if (ci != R.ci) w = grid[ci];
             R.ci = ci;
             // two or three axes step together (rare): best_t sums their t. (A
             // lane without a ray holds distinct t_max values, so it never
@@ -1357,13 +1364,10 @@ k_march(MarchArgs A) {
             else if (u == STEPS - 1) add_masked(R.meta, (uint32_t)STEPS << 20, run);
             uint64_t ok = __ballot(bt < R.maxt);
             if (u == STEPS - 1) ok &= __ballot((int32_t)R.meta >= 0);
-            // grid words are bit-reversed: voxel bit b sits at bit 63 - b. An
-            // occupied voxel stops the ray there; its step count (counted
+            // An occupied voxel stops the ray there; its step count (counted
             // here) loses that step when it finishes.
-            const bool occ = (int64_t)(w << b) < 0;
-            run = run & ~__ballot(occ) & ok;
+            run = run & ~occm & ok;
             advance_tmax(R, m0, m1, m2, run, full);
-            w = nw;
         }
         R.cw = w;
         running = (run & lane_bit) != 0;
