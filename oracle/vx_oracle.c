@@ -489,6 +489,14 @@ static inline void mat_vec(const float m[16], const float v[4], float out[4]) {
         out[r] = ((m[0 + r] * v[0] + m[4 + r] * v[1]) + m[8 + r] * v[2]) + m[12 + r] * v[3];
 }
 
+/* a * b, column-major, each entry summed in the order of mat_vec. */
+static void mat_mul_f(const float* a, const float* b, float* out) {
+    for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 4; ++r)
+            out[j * 4 + r] = ((a[0 + r] * b[j * 4 + 0] + a[4 + r] * b[j * 4 + 1]) + a[8 + r] * b[j * 4 + 2]) +
+                             a[12 + r] * b[j * 4 + 3];
+}
+
 /* Build-defined pinhole ray through pixel (px,py) from the ViewData inverse
  * matrices (LightAmbient.vert:32-36 FarVec, then the inverse view). */
 static void pixel_ray(const vxo_view* vw, int32_t px, int32_t py, float dir[3], float ndc[2]) {
@@ -503,18 +511,46 @@ static void pixel_ray(const vxo_view* vw, int32_t px, int32_t py, float dir[3], 
     normalize3(w, dir);
 }
 
+/* A draw's box [0, dims] (model space, fine voxels) rasterizes at a pixel when
+ * the pixel's forward ray (o, d) leaves it at a view depth >= NEAR: the CUBE
+ * triangles (GeometryVoxel.vert:5-42) face inward and back faces are culled
+ * with CCW fronts (evk.cpp:470-471), so the far faces are the ones drawn,
+ * clipped by the near plane. world maps model space (in tenths) to world
+ * space; NULL is the identity. pv = projection * view. */
+static int box_far_face_drawn(const int32_t dims[3], const float o[3], const float d[3], const float* world,
+                              const float* pv) {
+    float tn = -INFINITY, tf = INFINITY;
+    for (int k = 0; k < 3; ++k) {
+        float inv = 1.0f / d[k];
+        float t0 = (0.0f - o[k]) * inv, t1 = ((float)dims[k] - o[k]) * inv;
+        tn = fmaxf(tn, fminf(t0, t1));
+        tf = fminf(tf, fmaxf(t0, t1));
+    }
+    if (!(tn <= tf && tf > 0.0f)) return 0;
+    float pe[4] = {(o[0] + d[0] * tf) * 0.1f, (o[1] + d[1] * tf) * 0.1f, (o[2] + d[2] * tf) * 0.1f, 1.0f}, we[4], ce[4];
+    if (world) mat_vec(world, pe, we);
+    else for (int k = 0; k < 4; ++k) we[k] = pe[k];
+    mat_vec(pv, we, ce);
+    return ce[3] >= 0.1f;
+}
+
 /* GeometryVoxel main (GeometryVoxel.frag:127-183) for the world model with an
  * identity WorldMatrix: localCameraPos = CameraPosition*10
- * (GeometryVoxel.vert:74); G-buffer gets the fp32 local hit point. */
-static void primary_pixel(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw, int32_t px,
-                          int32_t py, float* pos, float* nrm, uint8_t* mat) {
+ * (GeometryVoxel.vert:74); G-buffer gets the fp32 local hit point. A pixel
+ * the model's box does not cover (box_far_face_drawn: no fragment, e.g. a
+ * box behind the camera) stays empty. pv = projection * view. */
+static void primary_pixel(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw, const float* pv,
+                          int32_t px, int32_t py, float* pos, float* nrm, uint8_t* mat) {
     float dir[3], ndc[2], cam[3], start[3], hit[3] = {0, 0, 0}, n[3] = {0, 0, 0};
     pixel_ray(vw, px, py, dir, ndc);
     for (int k = 0; k < 3; ++k) cam[k] = vw->cam_pos[k] * 10.0f;
     float size[3] = {(float)dims[0], (float)dims[1], (float)dims[2]};
-    vxo_clip_to_aabb(cam, dir, size, start);
     uint8_t m = 0;
-    int h = vxo_intersect_volume(mips, dims, start, dir, cam, ndc, vw->res, vw->frame, hit, n, &m);
+    int h = 0;
+    if (box_far_face_drawn(dims, cam, dir, NULL, pv)) {
+        vxo_clip_to_aabb(cam, dir, size, start);
+        h = vxo_intersect_volume(mips, dims, start, dir, cam, ndc, vw->res, vw->frame, hit, n, &m);
+    }
     if (!h) m = 0;
     for (int k = 0; k < 3; ++k) { pos[k] = h ? hit[k] : 0.0f; nrm[k] = h ? n[k] : 0.0f; }
     *mat = m;
@@ -523,6 +559,8 @@ static void primary_pixel(const uint8_t* const mips[3], const int32_t dims[3], c
 void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw,
                  float* pos, float* nrm, uint8_t* mat, int32_t nthreads) {
     const int32_t W = vw->width, H = vw->height;
+    float pv[16];
+    mat_mul_f(vw->proj, vw->view, pv);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
@@ -530,7 +568,7 @@ void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_
     for (int32_t py = 0; py < H; ++py) {
         for (int32_t px = 0; px < W; ++px) {
             size_t i = (size_t)py * W + px;
-            primary_pixel(mips, dims, vw, px, py, pos + 3 * i, nrm + 3 * i, mat + i);
+            primary_pixel(mips, dims, vw, pv, px, py, pos + 3 * i, nrm + 3 * i, mat + i);
         }
     }
     (void)nthreads;
@@ -539,14 +577,16 @@ void vxo_primary(const uint8_t* const mips[3], const int32_t dims[3], const vxo_
 void vxo_primary_pixels(const uint8_t* const mips[3], const int32_t dims[3], const vxo_view* vw,
                         const int32_t* pixels, int32_t npixels, float* pos, float* nrm, uint8_t* mat,
                         int32_t nthreads) {
+    float pv[16];
+    mat_mul_f(vw->proj, vw->view, pv);
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
 #endif
     for (int32_t j = 0; j < npixels; ++j) {
         const int32_t i = pixels[j];
-        primary_pixel(mips, dims, vw, i % vw->width, i / vw->width, pos + 3 * (size_t)j, nrm + 3 * (size_t)j,
-                      mat + j);
+        primary_pixel(mips, dims, vw, pv, i % vw->width, i / vw->width, pos + 3 * (size_t)j,
+                      nrm + 3 * (size_t)j, mat + j);
     }
     (void)nthreads;
 }
@@ -593,30 +633,8 @@ static int inverse4(const double* m, double* inv) {
     return 1;
 }
 
-static void mat_mul_f(const float* a, const float* b, float* out) {
-    for (int j = 0; j < 4; ++j)
-        for (int r = 0; r < 4; ++r)
-            out[j * 4 + r] = ((a[0 + r] * b[j * 4 + 0] + a[4 + r] * b[j * 4 + 1]) + a[8 + r] * b[j * 4 + 2]) +
-                             a[12 + r] * b[j * 4 + 3];
-}
-
-/* The draw's box rasterizes at the pixel when the forward ray leaves it at a
- * view depth >= NEAR: the CUBE triangles (GeometryVoxel.vert:5-42) face
- * inward and back faces are culled with CCW fronts (evk.cpp:470-471), so the
- * far faces are the ones drawn, clipped by the near plane. */
 static int box_covers(const vxo_vox_cmd* c, const float o[3], const float d[3], const float* pv) {
-    float tn = -INFINITY, tf = INFINITY;
-    for (int k = 0; k < 3; ++k) {
-        float inv = 1.0f / d[k];
-        float t0 = (0.0f - o[k]) * inv, t1 = ((float)c->dims[k] - o[k]) * inv;
-        tn = fmaxf(tn, fminf(t0, t1));
-        tf = fminf(tf, fmaxf(t0, t1));
-    }
-    if (!(tn <= tf && tf > 0.0f)) return 0;
-    float pe[4] = {(o[0] + d[0] * tf) * 0.1f, (o[1] + d[1] * tf) * 0.1f, (o[2] + d[2] * tf) * 0.1f, 1.0f}, we[4], ce[4];
-    mat_vec(c->world, pe, we);
-    mat_vec(pv, we, ce);
-    return ce[3] >= 0.1f;
+    return box_far_face_drawn(c->dims, o, d, c->world, pv);
 }
 
 /* The geometry pass per pixel: GeometryVoxel.vert:72-75 gives the camera in

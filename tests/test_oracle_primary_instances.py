@@ -179,3 +179,33 @@ def test_draws_behind_the_camera_cover_nothing_and_a_box_around_it_is_drawn(orac
     around["mips"] = oracle.build_mips(around["mips"][0])
     r = oracle.primary_instances([around], vd)
     assert (r["mat"] == 44).mean() > 0.2 and not (r["mat"] == 33).any()
+
+
+def test_single_model_pass_draws_only_where_the_box_is_drawn(oracle):
+    """The single-model pass is the same draw with an identity WorldMatrix, so
+    it covers the pixels the instance pass covers: none when the camera faces
+    away from the box (clipToAABB alone would start the march on the box
+    behind the camera and hit it), the far faces when the camera sits inside
+    the box, and one identity draw's pixels when the box is partly behind the
+    camera's plane."""
+    vol = _model(4)
+    n = vol.shape[0] / 10.0
+    mips = oracle.build_mips(vol)
+    cases = [((-0.5, 4.0, -0.5), (-2.5, 5.0, -2.5), "facing away"),
+             ((-0.5, 4.0, -0.5), (-0.5, 5.5, -3.0), "facing up and away"),
+             ((0.5 * n, 0.5 * n, 0.5 * n), (0.9 * n, 0.3 * n, 0.2 * n), "inside"),
+             ((0.5 * n, 0.6 * n, -0.3), (-2.0, 0.6 * n, 0.1), "beside, half behind")]
+    for eye, tgt, what in cases:
+        vd = views.make_view(eye, tgt, W, H)
+        pos, nrm, mat = oracle.primary(mips, vd)
+        r = oracle.primary_instances([_cmd(oracle, vol, np.eye(4))], vd)
+        hit = mat > 0
+        if what.startswith("facing"):
+            assert not hit.any(), what
+            assert not pos.any() and not nrm.any(), what
+        else:
+            assert hit.any(), what
+        # the same coverage and voxels as the identity draw (up to one
+        # renormalisation of the direction in the instance pass)
+        assert np.mean((r["mat"] > 0) == hit) > 0.995, what
+        assert np.mean(r["mat"] == mat) > 0.995, what

@@ -1044,6 +1044,14 @@ extern "C" vx_status vx_volume_download_level(const vx_volume* v, int32_t level,
 // ---------------------------------------------------------------------------
 // passes
 // ---------------------------------------------------------------------------
+// a * b in fp32, column j = ((a0*bj.x + a1*bj.y) + a2*bj.z) + a3*bj.w.
+static void mat_mul_f32(const float* a, const float* b, float* out) {
+    for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 4; ++r)
+            out[j * 4 + r] = ((a[0 + r] * b[j * 4 + 0] + a[4 + r] * b[j * 4 + 1]) + a[8 + r] * b[j * 4 + 2]) +
+                             a[12 + r] * b[j * 4 + 3];
+}
+
 extern "C" vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_view* view, const vx_gbuf* g,
                                 void* stream) {
     VX_REQUIRE(ctx && model && g, "ctx/model/gbuf is NULL");
@@ -1053,6 +1061,7 @@ extern "C" vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_vie
     PrimaryArgs A;
     A.view = *view;
     A.tiles = tile_map(view->width, view->height);
+    mat_mul_f32(view->proj, view->view, A.pv);
     for (int k = 0; k < 3; ++k) { A.mips[k] = model->mip[k]; A.dims[k] = model->dims[k]; }
     A.pos = g->pos; A.nrm = g->normal; A.mat = g->material;
     hipLaunchKernelGGL(k_primary, dim3(8 * A.tiles.per_xcd), dim3(256), 0, S(stream), A);
@@ -1100,14 +1109,6 @@ static bool inverse4(const double* m, double* inv) {
     if (det == 0.0 || !std::isfinite(det)) return false;
     for (int i = 0; i < 16; ++i) inv[i] = c[i] / det;
     return true;
-}
-
-// a * b in fp32, column j = ((a0*bj.x + a1*bj.y) + a2*bj.z) + a3*bj.w.
-static void mat_mul_f32(const float* a, const float* b, float* out) {
-    for (int j = 0; j < 4; ++j)
-        for (int r = 0; r < 4; ++r)
-            out[j * 4 + r] = ((a[0 + r] * b[j * 4 + 0] + a[4 + r] * b[j * 4 + 1]) + a[8 + r] * b[j * 4 + 2]) +
-                             a[12 + r] * b[j * 4 + 3];
 }
 
 // Conservative pixel rectangle of a draw's box (x0 > x1: nothing visible).

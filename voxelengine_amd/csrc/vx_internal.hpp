@@ -215,6 +215,7 @@ struct TaaArgs {
 struct PrimaryArgs {
     vx_view view;
     TileMap tiles;
+    float pv[16];             // projection * view (the box coverage test)
     const uint8_t* mips[3];
     int32_t dims[3];
     float* pos;

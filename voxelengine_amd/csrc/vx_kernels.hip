@@ -2169,6 +2169,35 @@ VXD f3 clip_to_aabb(f3 o, f3 d, const int32_t dims[3]) {
     return mk3(co[0] + cd[0] * (tmin - 0.001f), co[1] + cd[1] * (tmin - 0.001f), co[2] + cd[2] * (tmin - 0.001f));
 }
 
+// Whether a draw's box [0, dims] (model space, fine voxels) rasterizes at a
+// pixel. The CUBE triangles (GeometryVoxel.vert:5-42) face inward and the
+// pipeline culls back faces with CCW fronts (evk.cpp:470-471), so the far
+// faces are drawn: the pixel is covered when its forward ray (o, d: slabs in
+// model voxel units from the local camera) leaves the box at a view depth >=
+// NEAR (the near-plane clip). world maps model space (in tenths) to world
+// space, nullptr for the identity; pv = projection * view.
+VXD bool box_far_face_drawn(const int32_t dims[3], f3 lc, f3 dl, const float* world, const float* pv) {
+    const float o[3] = {lc.x, lc.y, lc.z}, d[3] = {dl.x, dl.y, dl.z};
+    float tn = -INFINITY, tf = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float inv = 1.0f / d[k];
+        const float t0 = (0.0f - o[k]) * inv, t1 = ((float)dims[k] - o[k]) * inv;
+        tn = fmaxf(tn, fminf(t0, t1));
+        tf = fminf(tf, fmaxf(t0, t1));
+    }
+    if (!(tn <= tf && tf > 0.0f)) return false;
+    float pe[4] = {(o[0] + d[0] * tf) * 0.1f, (o[1] + d[1] * tf) * 0.1f, (o[2] + d[2] * tf) * 0.1f, 1.0f}, we[4], ce[4];
+    if (world) {
+        mat_vec(world, pe, we);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) we[k] = pe[k];
+    }
+    mat_vec(pv, we, ce);
+    return ce[3] >= 0.1f;
+}
+
 __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A) {
     int32_t px, py;
     if (!tile_pixel(A.tiles, px, py)) return;
@@ -2180,10 +2209,15 @@ __global__ void __launch_bounds__(256) k_primary(PrimaryArgs A) {
     ModelMips M;
 #pragma unroll
     for (int k = 0; k < 3; ++k) { M.mips[k] = A.mips[k]; M.dims[k] = A.dims[k]; }
-    f3 start = clip_to_aabb(cam, dir, M.dims);
     f3 hit = mk3(0.0f, 0.0f, 0.0f), nrm = mk3(0.0f, 0.0f, 0.0f);
     uint32_t mat = 0;
-    int32_t h = intersect_volume(M, vw.res, vw.frame, start, dir, cam, ndc_x, ndc_y, hit, nrm, mat);
+    int32_t h = 0;
+    // a pixel the box does not cover (no fragment: e.g. the box is behind
+    // the camera) stays empty
+    if (box_far_face_drawn(M.dims, cam, dir, nullptr, A.pv)) {
+        f3 start = clip_to_aabb(cam, dir, M.dims);
+        h = intersect_volume(M, vw.res, vw.frame, start, dir, cam, ndc_x, ndc_y, hit, nrm, mat);
+    }
     if (!h) {
         hit = mk3(0.0f, 0.0f, 0.0f);
         nrm = mk3(0.0f, 0.0f, 0.0f);
@@ -2219,26 +2253,9 @@ __global__ void __launch_bounds__(256) k_bin_instances(BinArgs A) {
     A.counts[t] = c;
 }
 
-// Whether the draw's box rasterizes at this pixel. The CUBE triangles
-// (GeometryVoxel.vert:5-42) face inward and the pipeline culls back faces
-// with CCW fronts (evk.cpp:470-471), so the far faces are drawn: the pixel is
-// covered when its forward ray leaves the box at a view depth >= NEAR (the
-// near-plane clip). Slabs in model voxel units from the local camera.
+// Whether the draw's box rasterizes at this pixel (box_far_face_drawn).
 VXD bool box_covers(const InstConst& I, f3 lc, f3 dl, const float* pv) {
-    const float o[3] = {lc.x, lc.y, lc.z}, d[3] = {dl.x, dl.y, dl.z};
-    float tn = -INFINITY, tf = INFINITY;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const float inv = 1.0f / d[k];
-        const float t0 = (0.0f - o[k]) * inv, t1 = ((float)I.m.dims[k] - o[k]) * inv;
-        tn = fmaxf(tn, fminf(t0, t1));
-        tf = fminf(tf, fmaxf(t0, t1));
-    }
-    if (!(tn <= tf && tf > 0.0f)) return false;
-    float pe[4] = {(o[0] + d[0] * tf) * 0.1f, (o[1] + d[1] * tf) * 0.1f, (o[2] + d[2] * tf) * 0.1f, 1.0f}, we[4], ce[4];
-    mat_vec(I.w, pe, we);
-    mat_vec(pv, we, ce);
-    return ce[3] >= 0.1f;
+    return box_far_face_drawn(I.m.dims, lc, dl, I.w, pv);
 }
 
 // D24 UNORM conversion of a fragment depth (clamped to the depth range).
