@@ -247,7 +247,10 @@ vx_status vx_volume_download_level(const vx_volume* v, int32_t level, uint64_t* 
 /* ---------------------------------------------------------------- passes
  * Primary hit: GeometryVoxelPipeline::Use (GeometryVoxelPipeline.h:48-70) +
  * GeometryVoxel.frag main/intersectVolume (GeometryVoxel.frag:63-183) for the
- * world model with an identity WorldMatrix. */
+ * world model with an identity WorldMatrix. A pixel is marched where the
+ * model's box is drawn (its far faces, GeometryVoxel.vert:5-42, at a view
+ * depth >= NEAR); elsewhere, e.g. where the box is behind the camera, it stays
+ * empty (position, normal and material 0). */
 vx_status vx_primary(vx_ctx* ctx, const vx_model* model, const vx_view* view, const vx_gbuf* gbuf,
                      void* stream);
 /* One voxel draw of the geometry pass (GeometryVoxelPipeline::Cmd,
