@@ -3,7 +3,10 @@ looking up and away from it): every pixel is sky, so the ray-parallel trace,
 its batched form, the light pass on both paths, the reflection pass and the
 screen-space occlusion get no work. Each must still end and write the sky
 values, bit for bit against the oracle, and a batch holding the sky view next
-to a view of the scene must give both items the bits of their own launches."""
+to a view of the scene must give both items the bits of their own launches.
+The primary pass's coverage (which pixels the model's box is drawn at) is
+also checked against the oracle for cameras beside, inside and below the
+box."""
 import numpy as np
 import pytest
 import torch
@@ -149,3 +152,31 @@ def test_light_reflection_and_occlusion_passes_of_a_sky_view(ctx, oracle, scene)
                                    d, light)
     assert_bitwise(t.cpu().numpy(), tref, "reflection t")
     assert_bitwise(out.cpu().numpy(), rref, "reflection out")
+
+
+CAMERAS = {
+    "beside, half behind": (lambda n: (0.5 * n[0], 0.6 * n[1], -0.3), lambda n: (-2.0, 0.6 * n[1], 0.1)),
+    "inside": (lambda n: (0.5 * n[0], 0.5 * n[1], 0.5 * n[2]), lambda n: (0.9 * n[0], 0.3 * n[1], 0.2 * n[2])),
+    "below, looking up": (lambda n: (0.4 * n[0], -0.5, 0.6 * n[2]), lambda n: (0.5 * n[0], 2.0 * n[1], 0.5 * n[2])),
+    "corner, facing the scene": (lambda n: (-0.1 * n[0], 1.1 * n[1], -0.1 * n[2]),
+                                 lambda n: (0.5 * n[0], 0.2 * n[1], 0.5 * n[2])),
+}
+
+
+@pytest.mark.parametrize("camera", list(CAMERAS), ids=[c.replace(", ", "-").replace(" ", "_") for c in CAMERAS])
+def test_primary_coverage_matches_oracle(ctx, oracle, scene, camera):
+    """Where the model's box is drawn decides which pixels are marched: the
+    kernel and the oracle agree bit for bit for cameras beside the box (part
+    of it behind the camera plane), inside it, below it and facing it."""
+    from voxelengine_amd import api, views
+    n = np.array(scene["model"].dims, np.float64) / 10.0
+    eye, tgt = CAMERAS[camera]
+    vd = views.make_view(np.array(eye(n)), np.array(tgt(n)), W, H, frame=1)
+    g = api.GBuffer(W, H)
+    api.primary(ctx, scene["model"], vd, g)
+    torch.cuda.synchronize()
+    pos, nrm, mat = oracle.primary(scene["mips"], vd)
+    assert_bitwise(g.material.cpu().numpy(), mat, f"{camera}: material")
+    assert_bitwise(g.pos.cpu().numpy(), pos, f"{camera}: pos")
+    assert_bitwise(g.normal.cpu().numpy(), nrm, f"{camera}: normal")
+    assert (mat > 0).any()
