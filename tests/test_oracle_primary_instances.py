@@ -209,3 +209,27 @@ def test_single_model_pass_draws_only_where_the_box_is_drawn(oracle):
         # renormalisation of the direction in the instance pass)
         assert np.mean((r["mat"] > 0) == hit) > 0.995, what
         assert np.mean(r["mat"] == mat) > 0.995, what
+
+
+def test_box_faces_nearer_than_the_near_plane_are_clipped(oracle):
+    """A camera inside the box, half a voxel (0.05) from its x = 0 face, in a
+    set voxel of the x = 0 wall. Facing that face, the centre pixels leave the
+    box through it at a view depth under NEAR (0.1): the near plane clips the
+    face, no fragment is drawn and the pixels stay empty, although the march
+    would hit the camera's own voxel at once. Facing the other way, the far
+    x = 32 face is drawn and the same march hits. The identity draw agrees."""
+    vol = np.zeros((32, 32, 32), np.uint8)
+    vol[:, :, 0] = 40                                     # the x = 0 wall, around the camera
+    vol[:, :, 31] = 60
+    mips = oracle.build_mips(vol)
+    eye = np.array([0.05, 1.6, 1.6])
+    for look in (-1.0, 1.0):
+        vd = views.make_view(eye, eye + np.array([look, -0.02, 0.0]), W, H)
+        pos, nrm, mat = oracle.primary(mips, vd)
+        r = oracle.primary_instances([_cmd(oracle, vol, np.eye(4))], vd)
+        centre = mat.reshape(H, W)[H // 2 - 4:H // 2 + 4, W // 2 - 4:W // 2 + 4]
+        if look < 0:
+            assert not mat.any()
+        else:
+            assert centre.all()
+        assert np.mean((r["mat"] > 0) == (mat > 0)) > 0.995
