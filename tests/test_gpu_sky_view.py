@@ -28,7 +28,8 @@ def scene(ctx, oracle):
     g = api.GBuffer(W, H)
     api.primary(ctx, model, sky_vd, g)
     torch.cuda.synchronize()
-    return dict(model=model, occ=occ, occ_h=oracle_occ(oracle, vol), mips=oracle.build_mips(vol), vd=sky_vd, g=g)
+    return dict(model=model, occ=occ, vol=vol, occ_h=oracle_occ(oracle, vol), mips=oracle.build_mips(vol), vd=sky_vd,
+                g=g)
 
 
 def test_sky_view_has_no_hit(scene, oracle):
@@ -163,15 +164,35 @@ CAMERAS = {
 }
 
 
+def _camera(scene, camera):
+    """Eye and target (world units) of a named camera. The inside camera sits
+    in the first candidate point whose voxel and 4-voxel neighbourhood are
+    empty (a camera inside a set voxel hits it at t = 0 everywhere, and the
+    passes' view vectors are then NaN)."""
+    n = np.array(scene["model"].dims, np.float64) / 10.0
+    eye, tgt = CAMERAS[camera]
+    e = np.array(eye(n))
+    if camera == "inside":
+        vol = scene["vol"]                                  # (z, y, x) fine voxels
+        for c in ([0.5, 0.5, 0.5], [0.5, 0.85, 0.5], [0.3, 0.9, 0.7], [0.7, 0.95, 0.3], [0.5, 0.97, 0.5]):
+            v = (np.array(c) * np.array(scene["model"].dims)).astype(int)
+            lo, hi = np.maximum(v - 4, 0), v + 5
+            if not vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]].any():
+                e = (v + 0.5) / 10.0
+                break
+        else:
+            raise AssertionError("no empty point for the inside camera")
+    return e, np.array(tgt(n))
+
+
 @pytest.mark.parametrize("camera", list(CAMERAS), ids=[c.replace(", ", "-").replace(" ", "_") for c in CAMERAS])
 def test_primary_coverage_matches_oracle(ctx, oracle, scene, camera):
     """Where the model's box is drawn decides which pixels are marched: the
     kernel and the oracle agree bit for bit for cameras beside the box (part
     of it behind the camera plane), inside it, below it and facing it."""
     from voxelengine_amd import api, views
-    n = np.array(scene["model"].dims, np.float64) / 10.0
-    eye, tgt = CAMERAS[camera]
-    vd = views.make_view(np.array(eye(n)), np.array(tgt(n)), W, H, frame=1)
+    eye, tgt = _camera(scene, camera)
+    vd = views.make_view(eye, tgt, W, H, frame=1)
     g = api.GBuffer(W, H)
     api.primary(ctx, scene["model"], vd, g)
     torch.cuda.synchronize()
@@ -180,3 +201,63 @@ def test_primary_coverage_matches_oracle(ctx, oracle, scene, camera):
     assert_bitwise(g.pos.cpu().numpy(), pos, f"{camera}: pos")
     assert_bitwise(g.normal.cpu().numpy(), nrm, f"{camera}: normal")
     assert (mat > 0).any()
+
+
+@pytest.mark.parametrize("camera", ["beside, half behind", "inside", "below, looking up"],
+                         ids=["beside-half_behind", "inside", "below-looking_up"])
+def test_passes_of_other_cameras_match_oracle(ctx, oracle, scene, camera):
+    """The passes after the primary hit for the same cameras: the D24 depth
+    (its range), screen-space occlusion, the light pass and the reflection
+    pass (whose reprojection and sky depend on where the camera sits), bit
+    for bit against the oracle."""
+    from voxelengine_amd import api, configs, views
+    n = np.array(scene["model"].dims, np.float64) / 10.0
+    eye, tgt = _camera(scene, camera)
+    vd = views.make_view(eye, tgt, W, H, frame=1)
+    npx = W * H
+    g = api.GBuffer(W, H)
+    api.primary(ctx, scene["model"], vd, g)
+    depth = torch.full((npx,), -1.0, dtype=torch.float32, device="cuda")
+    api.gbuf_depth(ctx, vd, g, depth, volume_rid=1)
+    s = torch.full((npx,), -1.0, dtype=torch.float32, device="cuda")
+    api.ssao(ctx, vd, g, depth, s, seed=3)
+    torch.cuda.synchronize()
+    pos, nrm, mat = g.pos.cpu().numpy(), g.normal.cpu().numpy(), g.material.cpu().numpy()
+    d = depth.cpu().numpy()
+    assert ((d >= 0) & (d <= 1)).all()
+    assert_bitwise(s.cpu().numpy(), oracle.ssao(vd, nrm, mat, d, seed=3), f"{camera}: ssao")
+
+    pal = configs.make_palette()
+    pal_d = torch.from_numpy(pal).cuda()
+    rng = np.random.default_rng(13)
+    ext = n
+    L = np.zeros(6, api.LIGHT_DTYPE)
+    for k in range(L.size):
+        L[k]["type"] = k % 2
+        L[k]["position"] = rng.uniform([0.1, 0.2, 0.1], [0.9, 0.8, 0.9]) * ext
+        L[k]["range"] = 0.6 * ext.max()
+        L[k]["color"] = rng.uniform(0.5, 3.0, 3)
+        L[k]["attenuation"] = 1.5
+        dd = rng.normal(size=3)
+        L[k]["direction"] = dd / np.linalg.norm(dd)
+        L[k]["angle"] = 0.7
+        L[k]["angle_attenuation"] = 1.0
+    base = rng.random((npx, 3)).astype(np.float32)
+    ref, ref_f = oracle.light_pass(scene["occ_h"], vd, L, oracle.light_params(1, 2, 11, 1), pos, nrm, mat, pal,
+                                   rgb=base)
+    rgb = torch.from_numpy(base.copy()).cuda()
+    fetches = torch.full((npx,), -1, dtype=torch.int32, device="cuda")
+    api.light_pass(ctx, scene["occ"], vd, g, pal_d, api.lights_to_device(L), L.size, rgb, point_march=1,
+                   spot_march=2, seed=11, frame=1, fetches=fetches)
+    light = rng.uniform(0, 3, (npx, 4)).astype(np.float32)
+    out = torch.full((npx, 4), -1.0, dtype=torch.float32, device="cuda")
+    t = torch.full((npx,), -1.0, dtype=torch.float32, device="cuda")
+    api.reflection(ctx, scene["occ"], vd, g, pal_d, torch.from_numpy(light).cuda(), depth, out, t=t, march=0,
+                   maxt=256.0, seed=2, frame=1)
+    torch.cuda.synchronize()
+    assert_bitwise(rgb.cpu().numpy(), ref, f"{camera}: light rgb")
+    assert_bitwise(fetches.cpu().numpy().astype(np.uint32), ref_f, f"{camera}: light fetches")
+    rref, tref = oracle.reflection(scene["occ_h"], vd, oracle.reflection_params(0, 256.0, 2, 1), pos, nrm, mat, pal,
+                                   d, light)
+    assert_bitwise(t.cpu().numpy(), tref, f"{camera}: reflection t")
+    assert_bitwise(out.cpu().numpy(), rref, f"{camera}: reflection out")
